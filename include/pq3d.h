@@ -1,0 +1,104 @@
+/*
+ * pq3d.h -- C ABI of libpq3d.so, the MI355X (gfx950) implementation of the
+ * PyQuake3D per-step hot path.
+ *
+ * Plain pointers and sizes only.  Unless stated otherwise, every pq3d_*
+ * function returns 0 on success and a negative status on failure, with a
+ * message in pq3d_last_error() (thread-local).  Pointers named d_* are device
+ * pointers (from pq3d_malloc); all other array pointers are host memory.
+ *
+ * Part 1 keeps the reference's ctypes signatures unchanged so that the
+ * reference's own binding (src/Hmatrix.py:28-58) loads this library in place
+ * of TDstressFS_C.so.
+ */
+#ifndef PQ3D_H
+#define PQ3D_H
+
+#include <stdbool.h>
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ------------------------------------------------------------------------
+ * 1. Drop-in triangular-dislocation stress entry points
+ * ---------------------------------------------------------------------- */
+
+/* Replaces TDstressFS_C (src/TDstressFS_C.cpp:2095-2152), bound at
+ * src/Hmatrix.py:32-43.  n receivers (X, Y, Z) x one TD (P1, P2, P3: 3 each);
+ * unit-less Burgers components Ss (strike), Ds (dip), Ts (tensile); Lame mu,
+ * lambda_; half space if jud_halfspace.  Writes stress[i*6+k], strain[i*6+k],
+ * k over [xx, yy, zz, xy, xz, yz].  Computed on the GPU.  No status return
+ * (as the reference); on failure a message goes to stderr and the outputs
+ * are NaN. */
+void TDstressFS_C(const double* X, const double* Y, const double* Z, size_t n, const double* P1,
+                  const double* P2, const double* P3, double Ss, double Ds, double Ts, double mu,
+                  double lambda_, bool jud_halfspace, double* stress, double* strain);
+
+/* Replaces TDstressEachSourceAtReceiver_C (src/TDstressFS_C.cpp:2155-2200),
+ * bound at src/Hmatrix.py:45-56.  One receiver x n_sources TDs; P*_flat are
+ * n_sources x 3 (C order); outputs n_sources x 6. */
+void TDstressEachSourceAtReceiver_C(double x, double y, double z, const double* P1_flat,
+                                    const double* P2_flat, const double* P3_flat, size_t n_sources,
+                                    double Ss, double Ds, double Ts, double mu, double lambda_,
+                                    bool jud_halfspace, double* stress_out, double* strain_out);
+
+/* Replaces TDstressFS_C1 (src/TDstressFS_C.cpp:570-609): full space, twelve
+ * separate n-vectors. */
+void TDstressFS_C1(const double* X, const double* Y, const double* Z, size_t n, const double* P1,
+                   const double* P2, const double* P3, double Ss, double Ds, double Ts, double mu,
+                   double lambda_, double* Sxx, double* Syy, double* Szz, double* Sxy, double* Sxz,
+                   double* Syz, double* Exx, double* Eyy, double* Ezz, double* Exy, double* Exz,
+                   double* Eyz);
+
+/* ------------------------------------------------------------------------
+ * 2. Runtime: errors, devices, memory, events
+ * ---------------------------------------------------------------------- */
+const char* pq3d_last_error(void);
+int pq3d_device_count(int* n);
+int pq3d_device_info(int dev, char* name, int name_len, int* cu_count, double* hbm_bytes,
+                     int* arch_major, int* arch_minor);
+int pq3d_malloc(void** d_ptr, size_t bytes);
+int pq3d_free(void* d_ptr);
+int pq3d_memcpy_h2d(void* d_dst, const void* src, size_t bytes);
+int pq3d_memcpy_d2h(void* dst, const void* d_src, size_t bytes);
+int pq3d_memcpy_d2d(void* d_dst, const void* d_src, size_t bytes);
+int pq3d_memset(void* d_dst, int value, size_t bytes);
+int pq3d_synchronize(void);
+int pq3d_event_create(void** ev);
+int pq3d_event_destroy(void* ev);
+int pq3d_event_record(void* ev);
+int pq3d_event_elapsed_ms(void* start, void* stop, float* ms);
+
+/* Matrix storage types */
+#define PQ3D_MAT_F64 0
+#define PQ3D_MAT_F32 1
+#define PQ3D_MAT_BF16 2
+
+/* ------------------------------------------------------------------------
+ * 3. Dense traction Green's functions
+ * ---------------------------------------------------------------------- */
+
+/* Six traction Green's functions A1s, A2s, Bs, A1d, A2d, Bd: receiver
+ * centroid (row) x source TD (column) with unit strike (Ss=1) or dip (Ds=1)
+ * slip, projected on the receiver frame [e1, e2, n] -- replaces
+ * QDsim_gpu.get_coreAB_mulproess / worker / GetTtstress
+ * (src/QDsim_gpu.py:1745-1878, :1400-1417) and, with index lists, the dense
+ * block evaluators of src/Hmatrix.py:477-591.
+ *   P1, P2, P3 : nsrc x 3 source vertices; xg: nrcv x 3; ev: nrcv x 9
+ *   rows, cols : optional index lists (nr, nc entries), NULL = all
+ *   d_out[6]   : device buffers; element (i, j) of the nr x nc block at
+ *                j*ld + i (source-major), out_type PQ3D_MAT_F64 or _F32
+ *   src_per_block: sources walked per workgroup (<= 0: default) */
+int pq3d_gf_assemble(const double* P1, const double* P2, const double* P3, long long nsrc,
+                     const double* xg, const double* ev, long long nrcv, double mu, double lam,
+                     int halfspace, int out_type, void* const* d_out, long long ld,
+                     const long long* rows, long long nr, const long long* cols, long long nc,
+                     int src_per_block);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* PQ3D_H */
