@@ -97,6 +97,40 @@ int pq3d_gf_assemble(const double* P1, const double* P2, const double* P3, long 
                      const long long* rows, long long nr, const long long* cols, long long nc,
                      int src_per_block);
 
+/* ------------------------------------------------------------------------
+ * 4. Dense quasi-dynamic simulation (device-resident state)
+ *
+ * Replaces QDsim_gpu.QDsim's GPU state and stepping
+ * (src/QDsim_gpu.py:893-1027): initGPUvariable -> create/set_field,
+ * simu_forwardGPU -> pq3d_dense_step, *_gpu.get() -> pq3d_dense_get_field.
+ * Fields (index order): Tno Tt1o Tt2o state slipv1 slipv2 slipv slip1 slip2
+ * slip Tt rake rake0 fric a b dc f0 shear_loading normal_loading P dPdt0
+ * AdotV1 AdotV2 dsigmadt (units as in the reference: MPa, m/s, m, rad).
+ * ---------------------------------------------------------------------- */
+int pq3d_dense_create(void** sim, long long n, int mat_type, int kc);
+int pq3d_dense_destroy(void* sim);
+/* six device matrices A1s A2s Bs A1d A2d Bd, source-major with leading dim *ld */
+int pq3d_dense_matrix_ptrs(void* sim, void** d_ptrs6, long long* ld);
+int pq3d_dense_set_field(void* sim, int field, const double* host_n);
+int pq3d_dense_get_field(void* sim, int field, double* host_n);
+int pq3d_dense_field_ptr(void* sim, int field, void** d_ptr);
+/* eta = mu / (2 Cs); err_mode 1 = GPU-path error estimate (QDsim_gpu.py:1238),
+ * 0 = CPU-path estimate (QDsim_gpu.py:1373) */
+int pq3d_dense_set_params(void* sim, double V0, double Vpl, double eta, int fix_tn, int err_mode);
+int pq3d_dense_set_time(void* sim, double t);
+/* out6 = time, next dt, last h, last dtnext, max slip rate, last error2 */
+int pq3d_dense_state(void* sim, double* out6, int* nrjct, int* status, long long* steps);
+/* traction rates from the current slip rates into AdotV1/AdotV2/dsigmadt */
+int pq3d_dense_matvec(void* sim);
+int pq3d_dense_time_matvec(void* sim, int reps, float* ms_per_launch);
+/* one accepted step; returns 1 with 'error: dt is too small' in
+ * pq3d_last_error() where the reference exits (QDsim_gpu.py:983-985) */
+int pq3d_dense_step(void* sim, double dttry, double* h, double* dtnext, int* nrjct);
+/* nsteps steps without host synchronisation (HIP graph of graph_steps steps);
+ * records (host, optional): nsteps x 4 = h, dtnext, time, max slip rate */
+int pq3d_dense_run(void* sim, long long nsteps, int graph_steps, double* records, int* status);
+int pq3d_dense_reset_ctrl(void* sim);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1,0 +1,583 @@
+"""Drop-in for the reference's ``QDsim_gpu`` module (src/QDsim_gpu.py).
+
+``QDsim(elelst, nodelst, fnamePara)`` keeps the reference's constructor,
+attributes and stepping calls:
+
+* ``initGPUvariable()``                          QDsim_gpu.py:893-939
+* ``simu_forwardGPU(dttry) -> (h, dtnext)``      QDsim_gpu.py:942-1027
+* ``simu_forward(dttry) -> (h, dtnext)``         QDsim_gpu.py:1029-1122
+* attributes ``Tno_gpu``, ``Tt1o_gpu``, ``Tt2o_gpu``, ``state_gpu``,
+  ``slipv1_gpu``, ``slipv2_gpu``, ``slipv_gpu``, ``slip_gpu``, ``slip1_gpu``,
+  ``slip2_gpu``, ``Tt_gpu``, ``rake_gpu``, ``fric_gpu`` ... with ``.get()``
+  (main_gpu.py:120-129).
+
+Everything runs on the GPU through libpq3d.so.  ``simu_forward`` -- the
+reference's CPU branch -- is the same device step with the CPU branch's error
+estimate (all three traction errors, QDsim_gpu.py:1373), after which the host
+attributes (``slipv1``, ``Tt1o``, ...) are refreshed, as that branch leaves them.
+
+The Green's functions are assembled on the GPU in FP64 (or FP32 storage with
+the ``GF precision: fp32`` parameter key, which the reference does not have)
+and cached as ``A1s.npy ... Bd.npy`` in the ``Corefunc directory``, in the
+reference's layout (float64, row = receiver, column = source,
+QDsim_gpu.py:188-245).  Not supported: dilatancy / pore pressure (``If
+Dilatancy: True`` raises).
+"""
+import ctypes
+import os
+import sys
+from collections import deque
+from math import sqrt
+
+import numpy as np
+
+from . import _lib, greenfunc, readmsh
+from ._lib import check
+
+FIELDS = ("Tno", "Tt1o", "Tt2o", "state", "slipv1", "slipv2", "slipv", "slip1", "slip2", "slip", "Tt", "rake",
+          "rake0", "fric", "a", "b", "dc", "f0", "shear_loading", "normal_loading", "P", "dPdt0",
+          "AdotV1", "AdotV2", "dsigmadt")
+FIELD_ID = {name: i for i, name in enumerate(FIELDS)}
+
+
+def _declare(L):
+    vp, d, ll, i = ctypes.c_void_p, ctypes.c_double, ctypes.c_longlong, ctypes.c_int
+    dp = ctypes.POINTER(ctypes.c_double)
+    ip = ctypes.POINTER(ctypes.c_int)
+    _lib._sig(L, "pq3d_dense_create", [ctypes.POINTER(vp), ll, i, i])
+    _lib._sig(L, "pq3d_dense_destroy", [vp])
+    _lib._sig(L, "pq3d_dense_matrix_ptrs", [vp, ctypes.POINTER(vp), ctypes.POINTER(ll)])
+    _lib._sig(L, "pq3d_dense_set_field", [vp, i, dp])
+    _lib._sig(L, "pq3d_dense_get_field", [vp, i, dp])
+    _lib._sig(L, "pq3d_dense_field_ptr", [vp, i, ctypes.POINTER(vp)])
+    _lib._sig(L, "pq3d_dense_set_params", [vp, d, d, d, i, i])
+    _lib._sig(L, "pq3d_dense_set_time", [vp, d])
+    _lib._sig(L, "pq3d_dense_state", [vp, dp, ip, ip, ctypes.POINTER(ll)])
+    _lib._sig(L, "pq3d_dense_matvec", [vp])
+    _lib._sig(L, "pq3d_dense_time_matvec", [vp, i, ctypes.POINTER(ctypes.c_float)])
+    _lib._sig(L, "pq3d_dense_step", [vp, d, dp, dp, ip])
+    _lib._sig(L, "pq3d_dense_run", [vp, ll, i, dp, ip])
+    _lib._sig(L, "pq3d_dense_reset_ctrl", [vp])
+
+
+_lib.register(_declare)
+
+
+def readPara0(fname):
+    """``key: value`` parameter file, values kept as strings (QDsim_gpu.py:90-99)."""
+    Para0 = {}
+    with open(fname, "r") as f:
+        for line in f:
+            if ":" in line:
+                tem = line.split(":")
+                Para0[tem[0].strip()] = tem[1].strip()
+    return Para0
+
+
+def find_boundary_edges_and_nodes(triangles):
+    """Edges used by one triangle only, and their nodes (QDsim_gpu.py:56-80)."""
+    from collections import defaultdict
+    edge_count = defaultdict(int)
+    for tri in triangles:
+        for e in ((tri[0], tri[1]), (tri[1], tri[2]), (tri[2], tri[0])):
+            edge_count[tuple(sorted(e))] += 1
+    boundary_edges = [e for e, c in edge_count.items() if c == 1]
+    nodes = set()
+    for e in boundary_edges:
+        nodes.update(e)
+    return boundary_edges, np.array(list(nodes))
+
+
+class DenseSim:
+    """Device-resident dense simulation (owns the six matrices and all fields)."""
+
+    def __init__(self, n, mat_type=_lib.MAT_F64, kc=256):
+        L = _lib.lib()
+        h = ctypes.c_void_p()
+        check(L.pq3d_dense_create(ctypes.byref(h), int(n), int(mat_type), int(kc)), "pq3d_dense_create")
+        self.h = h
+        self.n = int(n)
+        self.mat_type = mat_type
+        ptrs = (ctypes.c_void_p * 6)()
+        ld = ctypes.c_longlong()
+        check(L.pq3d_dense_matrix_ptrs(h, ptrs, ctypes.byref(ld)), "matrix_ptrs")
+        self.mat_ptrs = [int(p) for p in ptrs]
+        self.ld = ld.value
+
+    def set(self, name, arr):
+        a = np.ascontiguousarray(np.broadcast_to(np.asarray(arr, dtype=np.float64), (self.n,)))
+        check(_lib.lib().pq3d_dense_set_field(self.h, FIELD_ID[name], _lib.ptr(a)), "set_field " + name)
+
+    def get(self, name):
+        out = np.empty(self.n)
+        check(_lib.lib().pq3d_dense_get_field(self.h, FIELD_ID[name], _lib.ptr(out)), "get_field " + name)
+        return out
+
+    def set_params(self, V0, Vpl, eta, fix_tn=False, gpu_semantics=True):
+        check(_lib.lib().pq3d_dense_set_params(self.h, V0, Vpl, eta, int(bool(fix_tn)), int(bool(gpu_semantics))),
+              "set_params")
+
+    def set_time(self, t):
+        check(_lib.lib().pq3d_dense_set_time(self.h, float(t)), "set_time")
+
+    def upload_matrix(self, k, A):
+        """Host (receiver x source) matrix -> device, converted to the storage type."""
+        A = np.asarray(A)
+        n = self.n
+        esz = {_lib.MAT_F64: 8, _lib.MAT_F32: 4, _lib.MAT_BF16: 2}[self.mat_type]
+        if self.mat_type == _lib.MAT_F64:
+            col = np.zeros((n, self.ld), dtype=np.float64)
+            col[:, :n] = A.T
+        elif self.mat_type == _lib.MAT_F32:
+            col = np.zeros((n, self.ld), dtype=np.float32)
+            col[:, :n] = A.T
+        else:
+            f = np.ascontiguousarray(A.T, dtype=np.float32)
+            bits = f.view(np.uint32)
+            rnd = ((bits >> 16) & 1) + 0x7FFF  # round to nearest even
+            col = np.zeros((n, self.ld), dtype=np.uint16)
+            col[:, :n] = ((bits + rnd) >> 16).astype(np.uint16)
+        check(_lib.lib().pq3d_memcpy_h2d(ctypes.c_void_p(self.mat_ptrs[k]), col.ctypes.data_as(ctypes.c_void_p),
+                                         n * self.ld * esz), "upload matrix")
+
+    def download_matrix(self, k):
+        """Device matrix -> host float64 (receiver x source)."""
+        n = self.n
+        dt = {_lib.MAT_F64: np.float64, _lib.MAT_F32: np.float32, _lib.MAT_BF16: np.uint16}[self.mat_type]
+        col = np.empty((n, self.ld), dtype=dt)
+        check(_lib.lib().pq3d_memcpy_d2h(col.ctypes.data_as(ctypes.c_void_p), ctypes.c_void_p(self.mat_ptrs[k]),
+                                         col.nbytes), "download matrix")
+        if dt == np.uint16:
+            col = (col.astype(np.uint32) << 16).view(np.float32)
+        return np.ascontiguousarray(col[:, :n].T, dtype=np.float64)
+
+    def assemble(self, P1, P2, P3, xg, ev, mu, lam, halfspace):
+        if self.mat_type == _lib.MAT_BF16:
+            raise ValueError("assemble into FP64 or FP32 storage")
+        greenfunc.assemble_device(P1, P2, P3, xg, ev, mu, lam, halfspace, self.mat_ptrs, self.ld,
+                                  self.mat_type)
+
+    def matvec(self):
+        check(_lib.lib().pq3d_dense_matvec(self.h), "matvec")
+
+    def time_matvec(self, reps=20):
+        f = ctypes.c_float()
+        check(_lib.lib().pq3d_dense_time_matvec(self.h, int(reps), ctypes.byref(f)), "time_matvec")
+        return f.value
+
+    def step(self, dttry):
+        h = ctypes.c_double()
+        dtn = ctypes.c_double()
+        nr = ctypes.c_int()
+        rc = _lib.lib().pq3d_dense_step(self.h, float(dttry), ctypes.byref(h), ctypes.byref(dtn), ctypes.byref(nr))
+        if rc == 1:
+            return None, None, nr.value
+        check(rc, "pq3d_dense_step")
+        return h.value, dtn.value, nr.value
+
+    def run(self, nsteps, graph_steps=8, records=False):
+        st = ctypes.c_int()
+        rec = np.zeros((nsteps, 4)) if records else None
+        check(_lib.lib().pq3d_dense_run(self.h, int(nsteps), int(graph_steps),
+                                        _lib.ptr(rec) if records else None, ctypes.byref(st)), "pq3d_dense_run")
+        return st.value, rec
+
+    def state(self):
+        out = np.zeros(6)
+        nr = ctypes.c_int()
+        stt = ctypes.c_int()
+        steps = ctypes.c_longlong()
+        check(_lib.lib().pq3d_dense_state(self.h, _lib.ptr(out), ctypes.byref(nr), ctypes.byref(stt),
+                                          ctypes.byref(steps)), "state")
+        return {"time": out[0], "dttry": out[1], "h": out[2], "dtnext": out[3], "maxV": out[4], "err2": out[5],
+                "nrjct": nr.value, "status": stt.value, "steps": steps.value}
+
+    def reset_ctrl(self):
+        check(_lib.lib().pq3d_dense_reset_ctrl(self.h), "reset_ctrl")
+
+    def close(self):
+        if self.h:
+            _lib.lib().pq3d_dense_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class DeviceField:
+    """A device-resident state vector; ``.get()`` copies it to the host."""
+
+    def __init__(self, sim, name):
+        self._sim = sim
+        self._name = name
+
+    def get(self):
+        return self._sim.get(self._name)
+
+    def set(self, arr):
+        self._sim.set(self._name, arr)
+
+    def __len__(self):
+        return self._sim.n
+
+    @property
+    def shape(self):
+        return (self._sim.n,)
+
+
+_GPU_ATTRS = {"Tno_gpu": "Tno", "Tt1o_gpu": "Tt1o", "Tt2o_gpu": "Tt2o", "state_gpu": "state",
+              "slipv1_gpu": "slipv1", "slipv2_gpu": "slipv2", "slipv_gpu": "slipv", "slip1_gpu": "slip1",
+              "slip2_gpu": "slip2", "slip_gpu": "slip", "Tt_gpu": "Tt", "rake_gpu": "rake",
+              "rake0_gpu": "rake0", "fric_gpu": "fric", "a_gpu": "a", "b_gpu": "b", "dc_gpu": "dc",
+              "f0_gpu": "f0", "shear_loading_gpu": "shear_loading", "normal_loading_gpu": "normal_loading",
+              "P_gpu": "P", "dPdt0_gpu": "dPdt0"}
+
+
+class QDsim:
+    """Dense quasi-dynamic earthquake-cycle simulator (QDsim_gpu.py:119-186)."""
+
+    def __init__(self, elelst, nodelst, fnamePara):
+        i = fnamePara.rfind("/")
+        self.dirname = fnamePara[:i] if i != -1 else fnamePara
+        self.Para0 = readPara0(fnamePara)
+        jud_ele_order = False
+        eleVec, xg = readmsh.get_eleVec(nodelst, elelst, jud_ele_order)
+        self.eleVec = eleVec
+        self.elelst = np.asarray(elelst)
+        self.nodelst = np.asarray(nodelst, dtype=np.float64)
+        self.xg = xg
+        self.maxslipvque = deque(maxlen=20)
+        self.val_nv = 0
+        P = self.Para0
+        self.mu = float(P["Shear modulus"])
+        self.lambda_ = float(P["Lame constants"])
+        self.density = float(P["Rock density"])
+        self.htry = 1e-3
+        self.Cs = sqrt(self.mu / self.density)
+        self.time = 0
+        self.halfspace_jud = P["Half space"] == "True"
+        self.InputHetoparamter = P["InputHetoparamter"] == "True"
+        self.useGPU = P.get("GPU", "True") == "True"
+        self.Ifdila = P.get("If Dilatancy", "False") == "True"
+        if self.Ifdila:
+            raise NotImplementedError("dilatancy / pore-pressure diffusion is not implemented "
+                                      "(QDsim_gpu.py:611-671)")
+        prec = P.get("GF precision", "fp64").lower()
+        self.mat_type = {"fp64": _lib.MAT_F64, "fp32": _lib.MAT_F32, "bf16": _lib.MAT_BF16}[prec]
+        YoungsM = self.mu * (3.0 * self.lambda_ + 2.0 * self.mu) / (self.lambda_ + self.mu)
+        possonratio = self.lambda_ / 2.0 / (self.lambda_ + self.mu)
+        print("Cs", self.Cs)
+        print("First Lamé constants", self.lambda_)
+        print("Shear Modulus", self.mu)
+        print("Youngs Modulus", YoungsM)
+        print("Poissons ratio", possonratio)
+        self.Relerrormax1_last = 0
+        self.Relerrormax2_last = 0
+        self.sim = DenseSim(len(elelst), self.mat_type)
+        self.Init_condition()
+        self.calc_corefunc()
+
+    # ---------------------------------------------------------------- GF
+    def calc_corefunc(self):
+        """Load the cached Green's functions or assemble them on the GPU
+        (QDsim_gpu.py:188-245)."""
+        directory = self.Para0["Corefunc directory"]
+        names = greenfunc.MATRIX_NAMES
+        have = os.path.exists(directory) and all(os.path.exists(os.path.join(directory, n + ".npy"))
+                                                 for n in names)
+        if not os.path.exists(directory):
+            os.mkdir(directory)
+        if have:
+            print("Start to load core functions...")
+            for k, n in enumerate(names):
+                self.sim.upload_matrix(k, np.load(os.path.join(directory, n + ".npy"), allow_pickle=False))
+            print("Core functions load completed.")
+        else:
+            print("Start to calculate core functions...")
+            P1, P2, P3 = greenfunc.source_vertices(self.nodelst, self.elelst)
+            if self.mat_type == _lib.MAT_BF16:
+                tmp = DenseSim(self.sim.n, _lib.MAT_F32)
+                tmp.assemble(P1, P2, P3, self.xg, self.eleVec, self.mu, self.lambda_, self.halfspace_jud)
+                for k in range(6):
+                    self.sim.upload_matrix(k, tmp.download_matrix(k))
+                tmp.close()
+            else:
+                self.sim.assemble(P1, P2, P3, self.xg, self.eleVec, self.mu, self.lambda_, self.halfspace_jud)
+            if self.Para0.get("save Corefunc", "True") == "True":
+                for k, n in enumerate(names):
+                    np.save(os.path.join(directory, n), self.sim.download_matrix(k))
+            print("Core functions computation completed")
+
+    def __getattr__(self, name):
+        # host copies of the matrices on demand (A1s, A2s, Bs, A1d, A2d, Bd)
+        if name in greenfunc.MATRIX_NAMES and "sim" in self.__dict__:
+            return self.sim.download_matrix(greenfunc.MATRIX_NAMES.index(name))
+        if name in _GPU_ATTRS and self.__dict__.get("_gpu_ready"):
+            return DeviceField(self.sim, _GPU_ATTRS[name])
+        raise AttributeError(name)
+
+    # ------------------------------------------------------ initial state
+    def calc_nucleaszie_cohesivezone(self):
+        """Element sizes, critical nucleation size and cohesive zone
+        (QDsim_gpu.py:282-317)."""
+        e = np.asarray(self.elelst) - 1
+        P1, P2, P3 = self.nodelst[e[:, 0]], self.nodelst[e[:, 1]], self.nodelst[e[:, 2]]
+        elesize = np.max(np.stack([np.linalg.norm(P1 - P2, axis=1), np.linalg.norm(P1 - P3, axis=1),
+                                   np.linalg.norm(P2 - P3, axis=1)], 1), axis=1)
+        maxsize = float(elesize.max())
+        self.maxsize = maxsize
+        self.ave_elesize = np.mean(elesize)
+        b = np.max(self.b)
+        a = np.min(self.a)
+        sigma = np.mean(self.Tno * 1e6)
+        L = np.max(self.dc)
+        print("L:", L)
+        self.hRA = 2.0 / np.pi * self.mu * b * L / (b - a) / (b - a) / sigma
+        self.hRA = self.hRA * np.pi * np.pi / 4.0
+        self.A0 = 9.0 * np.pi / 32 * self.mu * L / (b * sigma)
+        print("maximum element size", maxsize)
+        print("average elesize", self.ave_elesize)
+        print("Critical nucleation size", self.hRA)
+        print("Cohesive zone:", self.A0)
+        return maxsize
+
+    def read_parameter(self, fname):
+        """10-column per-element initial state (QDsim_gpu.py:714-770)."""
+        values = np.loadtxt(os.path.join(self.dirname, fname), ndmin=2)
+        Ncell = self.eleVec.shape[0]
+        self.rake = values[:Ncell, 0]
+        self.a = values[:Ncell, 1]
+        self.b = values[:Ncell, 2]
+        self.dc = values[:Ncell, 3]
+        self.f0 = values[:Ncell, 4]
+        self.Tt = values[:Ncell, 5]
+        self.Tno = values[:Ncell, 6]
+        self.slipv = values[:Ncell, 7]
+        self.shear_loading = values[:Ncell, 8]
+        self.normal_loading = values[:Ncell, 9]
+        x = np.cos(self.rake)
+        y = np.sin(self.rake)
+        self.slipv1 = self.slipv * x + 1e-16
+        self.slipv2 = self.slipv * y + 1e-16
+        self.vec_Tra = np.array([x, y]).transpose()
+        self.Tt = (self.Tno - self.P * 1e-6) * self.a * np.arcsinh(
+            self.slipv / (2.0 * self.V0) * np.exp((self.f0 + self.b * np.log(self.V0 / self.slipvC)) / self.a))
+        self.Tt1o = self.Tt * x
+        self.Tt2o = self.Tt * y
+        self.fric = self.Tt / (self.Tno - self.P * 1e-6)
+        self.state = np.log(np.sinh(self.Tt / (self.Tno - self.P * 1e-6) / self.a) * 2.0 * self.V0 / self.slipv) * self.a
+
+    def Init_condition(self):
+        """Initial stress, friction parameters and state (QDsim_gpu.py:319-609)."""
+        P = self.Para0
+        N = len(self.eleVec)
+        self.Tt1o = np.zeros(N)
+        self.Tt2o = np.zeros(N)
+        self.Tno = np.zeros(N)
+        self.fix_Tn = P["Fix_Tn"] == "True"
+        ssv_scale = float(P["ssv_scale"])
+        ssh1_scale = float(P["ssh1_scale"])
+        ssv0_scale = float(P["ssh2_scale"])
+        trac_nor = float(P["Vertical principal stress value"])
+        self.P0 = 0
+        self.P = np.zeros(N)
+        self.dPdt0 = np.zeros(N)
+        self.local_index = np.arange(0, N, 1)
+        solve_normal = P["Normal traction solved from stress tensor"] == "True"
+        depth_var = P["Vertical principal stress value varies with depth"] == "True"
+        Ph1ang = np.pi / 180. * float(P["Angle between ssh1 and X-axis"])
+        Rm = np.array([[np.cos(Ph1ang), -np.sin(Ph1ang)], [np.sin(Ph1ang), np.cos(Ph1ang)]])
+        for i in range(N):
+            if depth_var:
+                turning_dep = float(P["Turnning depth"])
+                ssv = -self.xg[i, 2] / turning_dep + 0.2
+                if ssv > 1.0:
+                    ssv = 1.0
+                ssv = trac_nor * ssv * ssv_scale
+            else:
+                ssv = trac_nor * ssv_scale
+            ssh1 = -ssv * ssh1_scale
+            ssh2 = -ssv * ssv0_scale
+            ssv = -ssv
+            stress = np.dot(np.dot(Rm, np.array([[ssh1, 0], [0, ssh2]])), Rm.transpose())
+            stress3D = np.array([[stress[0][0], stress[0][1], 0], [stress[1][0], stress[1][1], 0], [0, 0, ssv]])
+            tra = np.dot(stress3D, self.eleVec[i, -3:])
+            ev = self.eleVec[i]
+            self.Tt1o[i] = tra[0] * ev[0] + tra[1] * ev[1] + tra[2] * ev[2]
+            self.Tt2o[i] = tra[0] * ev[3] + tra[1] * ev[4] + tra[2] * ev[5]
+            self.Tno[i] = tra[0] * ev[6] + tra[1] * ev[7] + tra[2] * ev[8]
+            if not solve_normal:
+                self.Tno[i] = ssv
+        self.Tno = np.abs(self.Tno)
+        self.Tt = np.sqrt(self.Tt1o * self.Tt1o + self.Tt2o * self.Tt2o)
+        with np.errstate(invalid="ignore", divide="ignore"):
+            x = self.Tt1o / self.Tt
+            y = self.Tt2o / self.Tt
+        solve_shear = P["Shear traction solved from stress tensor"] == "True"
+        if P["Rake solved from stress tensor"] == "True":
+            self.rake = np.arctan2(y, x)
+        else:
+            self.rake = np.ones(len(self.Tt)) * float(P["Fix_rake"]) / 180.0 * np.pi
+        if not solve_normal and not depth_var:
+            self.Tno = np.ones(len(self.Tno)) * trac_nor
+        self.vec_Tra = np.array([x, y]).transpose()
+        self.Vpl_con = float(P["Plate loading rate"])
+        self.slipvC = np.ones(len(self.xg)) * self.Vpl_con
+        self.shear_loading = np.zeros(N)
+        self.normal_loading = np.zeros(N)
+        self.V0 = float(P["Reference slip rate"])
+        self.dc = np.ones(N) * 0.02
+        self.f0 = np.ones(N) * float(P["Reference friction coefficient"])
+        self.a = np.zeros(N)
+        self.b = np.ones(N) * 0.03
+        self.slipv1 = np.zeros(N)
+        self.slipv2 = np.zeros(N)
+        # the reference aliases slipv and slipvC here (QDsim_gpu.py:446): writes
+        # to slipv in the nucleation patch also change slipvC
+        self.slipv = self.slipvC
+        self.slip1 = np.zeros(N)
+        self.slip2 = np.zeros(N)
+        self.slip = np.zeros(N)
+        self.arriT = np.ones(N) * 1e9
+        if self.InputHetoparamter:
+            self.read_parameter(P["Inputparamter file"])
+            self.maxslipv0 = np.max(self.slipv)
+            self.rake0 = np.copy(self.rake)
+            self.calc_nucleaszie_cohesivezone()
+            return
+        self._zone_init(solve_shear)
+
+    def _zone_init(self, solve_shear):
+        """Zone-based friction parameters (QDsim_gpu.py:473-609)."""
+        P = self.Para0
+        N = len(self.eleVec)
+        _, boundary_nodes = find_boundary_edges_and_nodes(self.elelst)
+        boundary_coord = self.nodelst[boundary_nodes - 1]
+        index_surface = np.where(np.abs(boundary_coord[:, 2] - 0.0) < 1e-5)[0]
+        index_sb = np.setdiff1d(np.arange(0, len(boundary_coord), 1), index_surface)
+        bsurf = boundary_coord[index_surface]
+        bsb = boundary_coord[index_sb]
+        nuclearloc = np.array([float(P["Nuclea_posx"]), float(P["Nuclea_posy"]), float(P["Nuclea_posz"])])
+        Wedge = float(P["Widths of VS region"])
+        Wedge_surface = float(P["Widths of surface VS region"])
+        transregion = float(P["Transition region from VS to VW region"])
+        aVs, bVs, dcVs = (float(P[k]) for k in ("Rate-and-state parameters a in VS region",
+                                                "Rate-and-state parameters b in VS region",
+                                                "Characteristic slip distance in VS region"))
+        aVw, bVw, dcVw = (float(P[k]) for k in ("Rate-and-state parameters a in VW region",
+                                                "Rate-and-state parameters b in VW region",
+                                                "Characteristic slip distance in VW region"))
+        aNu, bNu, dcNu = (float(P[k]) for k in ("Rate-and-state parameters a in nucleation region",
+                                                "Rate-and-state parameters b in nucleation region",
+                                                "Characteristic slip distance in nucleation region"))
+        slivpNu = float(P["Initial slip rate in nucleation region"])
+        Set_nuclear = P["Set_nucleation"] == "True"
+        Radiu_nuclear = float(P["Radius of nucleation"])
+        ChangefriA = P["ChangefriA"] == "True"
+        T_globalarr = []
+        for i in range(N):
+            distem = np.min(np.linalg.norm(bsb - self.xg[i], axis=1))
+            dis = distem / Wedge
+            dis_surface = dis
+            dis1 = (distem - Wedge) / transregion
+            if len(bsurf) > 10:
+                distem_surface = np.min(np.linalg.norm(bsurf - self.xg[i], axis=1))
+                dis_surface = distem_surface / Wedge_surface
+                dis1 = min(distem - Wedge, distem_surface - Wedge_surface) / transregion
+            if dis < 1.0 or dis_surface < 1.0:
+                self.a[i], self.b[i], self.dc[i] = aVs, bVs, dcVs
+            elif dis1 < 1.0:
+                if ChangefriA:
+                    self.a[i] = aVs - (aVs - aVw) * dis1
+                    self.b[i] = bVs
+                else:
+                    self.a[i] = aVs
+                    self.b[i] = bVs - (bVs - bVw) * dis1
+                self.dc[i] = dcVs
+            else:
+                self.a[i], self.b[i], self.dc[i] = aVw, bVw, dcVw
+            if np.linalg.norm(self.xg[i] - nuclearloc) < Radiu_nuclear and Set_nuclear:
+                self.slipv[i] = slivpNu
+                self.dc[i], self.a[i], self.b[i] = dcNu, aNu, bNu
+            T_local = np.array([np.cos(self.rake[i]), np.sin(self.rake[i]), 0.0])
+            T_globalarr.append(np.dot(self.eleVec[i].reshape([3, 3]).transpose(), T_local))
+        self.T_globalarr = np.array(T_globalarr)
+        self.slipv1 = self.slipv * np.cos(self.rake) + 1e-16
+        self.slipv2 = self.slipv * np.sin(self.rake) + 1e-16
+        if not solve_shear:
+            self.Tt = self.Tno * self.a * np.arcsinh(
+                self.slipv / (2.0 * self.V0) * np.exp((self.f0 + self.b * np.log(self.V0 / self.slipvC)) / self.a))
+        self.vec_Tra = np.array([np.cos(self.rake), np.sin(self.rake)]).transpose()
+        self.fric = self.Tt / self.Tno
+        self.state = np.log(np.sinh(self.Tt / self.Tno / self.a) * 2.0 * self.V0 / self.slipv) * self.a
+        self.Tt1o = self.Tt * np.cos(self.rake)
+        self.Tt2o = self.Tt * np.sin(self.rake)
+        self.maxslipv0 = np.max(self.slipv)
+        self.rake0 = np.copy(self.rake)
+        self.calc_nucleaszie_cohesivezone()
+
+    # ----------------------------------------------------------- stepping
+    def initGPUvariable(self):
+        """Upload the state and parameters (QDsim_gpu.py:893-939)."""
+        s = self.sim
+        for name in ("Tno", "Tt1o", "Tt2o", "state", "slipv1", "slipv2", "slipv", "slip1", "slip2", "slip", "Tt",
+                     "rake", "fric", "a", "b", "dc", "f0", "shear_loading", "normal_loading", "P", "dPdt0"):
+            s.set(name, getattr(self, name))
+        s.set("rake0", self.rake)
+        s.set_params(self.V0, self.Vpl_con, self.mu / (2.0 * self.Cs), self.fix_Tn, True)
+        s.set_time(self.time)
+        self._gpu_ready = True
+
+    def _step(self, dttry, gpu_semantics):
+        if not self.__dict__.get("_gpu_ready"):
+            self.initGPUvariable()
+        self.sim.set_params(self.V0, self.Vpl_con, self.mu / (2.0 * self.Cs), self.fix_Tn, gpu_semantics)
+        self.sim.set_time(self.time)
+        h, dtnext, nrjct = self.sim.step(dttry)
+        if h is None:
+            print("error: dt is too small")
+            sys.exit()
+        st = self.sim.state()
+        self.Relerrormax2 = st["err2"]
+        self.time = self.time + h
+        self.maxslipv0 = st["maxV"]
+        return h, dtnext
+
+    def simu_forwardGPU(self, dttry):
+        """One accepted step on the device; returns (h, dtnext)."""
+        return self._step(dttry, True)
+
+    def simu_forward(self, dttry):
+        """The CPU branch's step semantics, computed on the device; refreshes the
+        host attributes the CPU branch maintains."""
+        h, dtnext = self._step(dttry, False)
+        for name in ("Tno", "Tt1o", "Tt2o", "state", "slipv1", "slipv2", "slipv", "slip1", "slip2", "slip", "Tt",
+                     "rake", "fric"):
+            setattr(self, name, self.sim.get(name))
+        return h, dtnext
+
+    def run(self, nsteps, dttry=None, graph_steps=8, records=True):
+        """``nsteps`` device-resident steps with no host synchronisation in
+        between (an extension for throughput; the reference steps one at a
+        time).  Returns the per-step log (h, dtnext, time, max slip rate)."""
+        if not self.__dict__.get("_gpu_ready"):
+            self.initGPUvariable()
+        self.sim.set_time(self.time)
+        first = None
+        if dttry is not None:  # the first step takes dttry; later ones take dtnext on the device
+            h, dtnext, _ = self.sim.step(dttry)
+            if h is None:
+                raise RuntimeError("error: dt is too small")
+            first = np.array([[h, dtnext, self.time + h, self.sim.state()["maxV"]]])
+            nsteps -= 1
+        status, rec = self.sim.run(nsteps, graph_steps, records)
+        if status != 0:
+            raise RuntimeError("error: dt is too small" if status == 1 else f"device status {status}")
+        st = self.sim.state()
+        self.time = st["time"]
+        if records and first is not None:
+            rec = np.concatenate([first, rec])
+        return rec

@@ -1,0 +1,197 @@
+"""GPU parity of the dense stepping path against the NumPy FP64 oracle.
+
+Both sides start from the same state and use the same FP64 Green's functions
+(assembled on the GPU, downloaded for the oracle).  Bars:
+* step sizes h, dtnext: relative 1e-9 over the first 120 steps;
+* state vectors: relative to their max magnitude, 1e-9 over those steps;
+* traction matvec: FP64 storage 1e-12 of sum|a_ij u_j| per row, FP32 storage
+  1e-6, BF16 storage 1e-2;
+* failure handling: a NaN forced into the state from the host gives the
+  reference's 'dt is too small' outcome with the state left unchanged.
+"""
+import os
+
+import numpy as np
+import pytest
+
+from oracle import qdsim_oracle as Q
+
+pytestmark = pytest.mark.gpu
+
+MU = LAM = 32038120320.0
+CS = 3464.0
+V0, VPL = 1e-6, 1e-9
+
+
+def _case(nx, nz, length, depth, dc_sigma=0.2, seed=3):
+    from pyquake3d_amd import readmsh, synth, greenfunc
+    nodes_km, tri = synth.planar_fault_mesh(nx, nz, length, depth)
+    nodes = nodes_km * 1e3
+    ev, xg = readmsh.get_eleVec(nodes, tri, False)
+    cols = synth.friction_params(xg, dc_sigma=dc_sigma, seed=seed)
+    P = greenfunc.source_vertices(nodes, tri)
+    return nodes, tri, ev, xg, cols, P
+
+
+def _device_sim(case, mat_type, gpu_semantics=True):
+    from pyquake3d_amd.QDsim_gpu import DenseSim
+    nodes, tri, ev, xg, cols, P = case
+    n = len(tri)
+    s = DenseSim(n, mat_type)
+    s.assemble(*P, xg, ev, MU, LAM, True)
+    orc = Q.from_init_columns([np.zeros((1, 1))] * 6, cols, V0, VPL, MU, CS, gpu_semantics)
+    for name in ("Tno", "Tt1o", "Tt2o", "state", "slipv1", "slipv2", "slipv", "a", "b", "dc", "f0",
+                 "shear_loading", "normal_loading"):
+        s.set(name, getattr(orc, name))
+    s.set("rake0", orc.rake0)
+    s.set_params(V0, VPL, MU / (2 * CS), False, gpu_semantics)
+    return s, orc
+
+
+@pytest.fixture(scope="module")
+def small_case():
+    return _case(20, 10, 40.0, 20.0)
+
+
+def test_matvec_precision_per_storage_type(small_case):
+    from pyquake3d_amd import _lib
+    rng = np.random.default_rng(0)
+    n = len(small_case[1])
+    V1 = rng.lognormal(-20, 3, n) * rng.choice([-1, 1], n)
+    V2 = rng.lognormal(-20, 3, n)
+    ref = None
+    for mt, tol in ((_lib.MAT_F64, 1e-12), (_lib.MAT_F32, 1e-6), (_lib.MAT_BF16, 1e-2)):
+        s, orc = _device_sim(small_case, _lib.MAT_F64 if mt == _lib.MAT_BF16 else mt)
+        if mt == _lib.MAT_BF16:
+            mats = [s.download_matrix(k) for k in range(6)]
+            s.close()
+            from pyquake3d_amd.QDsim_gpu import DenseSim
+            s = DenseSim(n, _lib.MAT_BF16)
+            for k in range(6):
+                s.upload_matrix(k, mats[k])
+            s.set("rake0", orc.rake0)
+            s.set_params(V0, VPL, MU / (2 * CS), False, True)
+        if ref is None:
+            ref = [s.download_matrix(k) for k in range(6)]
+        s.set("slipv1", V1)
+        s.set("slipv2", V2)
+        s.matvec()
+        u1 = V1 - VPL * np.cos(orc.rake0)
+        u2 = V2 - VPL * np.sin(orc.rake0)
+        want = [ref[0] @ u1 + ref[3] @ u2, ref[1] @ u1 + ref[4] @ u2, ref[2] @ u1 + ref[5] @ u2]
+        mag = [np.abs(ref[0]) @ np.abs(u1) + np.abs(ref[3]) @ np.abs(u2),
+               np.abs(ref[1]) @ np.abs(u1) + np.abs(ref[4]) @ np.abs(u2),
+               np.abs(ref[2]) @ np.abs(u1) + np.abs(ref[5]) @ np.abs(u2)]
+        for name, w, m in zip(("AdotV1", "AdotV2", "dsigmadt"), want, mag):
+            got = s.get(name)
+            err = np.abs(got - w) / np.where(m > 0, m, 1.0)
+            assert err.max() < tol, (mt, name, err.max())
+        s.close()
+
+
+@pytest.mark.parametrize("gpu_semantics", [True, False])
+def test_step_sequence_matches_oracle(small_case, gpu_semantics):
+    from pyquake3d_amd import _lib
+    s, orc = _device_sim(small_case, _lib.MAT_F64, gpu_semantics)
+    orc.A1s, orc.A2s, orc.Bs, orc.A1d, orc.A2d, orc.Bd = [s.download_matrix(k) for k in range(6)]
+    dt = 1e-3
+    for k in range(120):
+        h, dtn, nr = s.step(dt)
+        ho, dto = orc.step(dt)
+        assert h is not None
+        assert abs(h - ho) <= 1e-9 * ho and abs(dtn - dto) <= 1e-9 * dto, (k, h, ho, dtn, dto)
+        assert nr == orc.nrjct
+        dt = dtn
+    for name in ("Tno", "Tt1o", "Tt2o", "state", "slipv1", "slipv2", "slipv", "slip1", "slip2", "slip", "Tt",
+                 "rake", "fric"):
+        got, want = s.get(name), getattr(orc, name)
+        scale = np.abs(want).max()
+        assert np.abs(got - want).max() <= 1e-9 * scale, (name, np.abs(got - want).max() / scale)
+    st = s.state()
+    assert st["steps"] == 120 and abs(st["time"] - orc.time) <= 1e-9 * orc.time
+    assert abs(st["maxV"] - orc.maxslipv0) <= 1e-9 * orc.maxslipv0
+    s.close()
+
+
+def test_rejection_matches_oracle(small_case):
+    from pyquake3d_amd import _lib
+    s, orc = _device_sim(small_case, _lib.MAT_F64)
+    orc.A1s, orc.A2s, orc.Bs, orc.A1d, orc.A2d, orc.Bd = [s.download_matrix(k) for k in range(6)]
+    # an elevated slip-rate patch makes a large first step fail the tolerance
+    V = orc.slipv.copy()
+    V[:40] = 1e-3
+    orc.slipv1 = V * np.cos(orc.rake0) + 1e-16
+    orc.slipv = np.sqrt(orc.slipv1 ** 2 + orc.slipv2 ** 2)
+    s.set("slipv1", orc.slipv1)
+    s.set("slipv", orc.slipv)
+    h, dtn, nr = s.step(1e5)
+    ho, dto = orc.step(1e5)
+    assert nr == orc.nrjct and nr >= 1
+    assert abs(h - ho) <= 1e-9 * ho and abs(dtn - dto) <= 1e-9 * dto
+    s.close()
+
+
+def test_forced_failure_leaves_state_unchanged(small_case):
+    from pyquake3d_amd import _lib
+    s, orc = _device_sim(small_case, _lib.MAT_F64)
+    st = orc.state.copy()
+    st[7] = np.nan
+    s.set("state", st)
+    before = s.get("Tt1o")
+    h, dtn, nr = s.step(1.0)
+    assert h is None and nr == 21
+    assert np.array_equal(s.get("Tt1o"), before)
+    assert s.state()["status"] == 1
+    s.close()
+
+
+def test_graph_run_equals_single_steps(small_case):
+    from pyquake3d_amd import _lib
+    a, _ = _device_sim(small_case, _lib.MAT_F32)
+    b, _ = _device_sim(small_case, _lib.MAT_F32)
+    h, dtn, _ = a.step(1e-3)
+    b.step(1e-3)
+    recs = []
+    for k in range(37):
+        h, dtn, _ = a.step(dtn)
+        recs.append((h, dtn))
+    status, rec = b.run(37, graph_steps=8, records=False)
+    assert status == 0
+    for name in ("state", "Tt1o", "slipv1", "slip"):
+        assert np.array_equal(a.get(name), b.get(name)), name
+    sa, sb = a.state(), b.state()
+    assert sa["time"] == sb["time"] and sa["dttry"] == sb["dttry"]
+    c, _ = _device_sim(small_case, _lib.MAT_F32)
+    c.step(1e-3)
+    status, rec = c.run(37, graph_steps=8, records=True)
+    assert status == 0 and np.array_equal(rec[:, :2], np.array(recs))
+    assert np.array_equal(c.get("state"), a.get("state"))
+    for s in (a, b, c):
+        s.close()
+
+
+def test_dropin_qdsim_api(tmp_path):
+    """The reference's driver pattern (main_gpu.py:106-136) on a synthetic case."""
+    from pyquake3d_amd import QDsim_gpu, readmsh, synth
+    msh, par = synth.write_case(str(tmp_path / "case"), 12, 6, 24.0, 12.0, dc_sigma=0.2, seed=1)
+    nodelst, elelst = readmsh.read_mshV2(msh)
+    sim0 = QDsim_gpu.QDsim(elelst, nodelst, par)
+    core = sim0.Para0["Corefunc directory"]
+    assert all(os.path.exists(os.path.join(core, n + ".npy")) for n in ("A1s", "A2s", "Bs", "A1d", "A2d", "Bd"))
+    sim0.initGPUvariable()
+    orc = Q.from_init_columns([np.load(os.path.join(core, n + ".npy")) for n in
+                               ("A1s", "A2s", "Bs", "A1d", "A2d", "Bd")],
+                              np.loadtxt(os.path.join(os.path.dirname(par), "param.dat")), V0, VPL, MU,
+                              np.sqrt(MU / sim0.density))
+    dttry = sim0.htry
+    for i in range(30):
+        dttry, dtnext = sim0.simu_forwardGPU(dttry)
+        ho, dto = orc.step(dttry if i == 0 else dto)
+        v1 = sim0.slipv1_gpu.get()
+        assert abs(dttry - ho) <= 1e-9 * ho
+        assert np.abs(v1 - orc.slipv1).max() <= 1e-9 * np.abs(orc.slipv1).max()
+        dttry = dtnext
+    assert abs(sim0.time - orc.time) <= 1e-9 * orc.time
+    # a second construction loads the cached Green's functions
+    sim1 = QDsim_gpu.QDsim(elelst, nodelst, par)
+    assert np.array_equal(sim1.A1s, sim0.A1s)
