@@ -131,6 +131,29 @@ int pq3d_dense_step(void* sim, double dttry, double* h, double* dtnext, int* nrj
 int pq3d_dense_run(void* sim, long long nsteps, int graph_steps, double* records, int* status);
 int pq3d_dense_reset_ctrl(void* sim);
 
+/* ------------------------------------------------------------------------
+ * 5. Ensemble: B = 64 simulations on one mesh (no reference counterpart;
+ *    the reference runs independent QDsim_gpu instances)
+ *
+ * One global step = one accepted RK45 step per active member, each with its
+ * own step size; the six FP32 matrices are streamed once per global step by a
+ * tall-skinny MFMA GEMM.  Field arrays are B x n (member-major), field
+ * indices as in section 4.
+ * ---------------------------------------------------------------------- */
+int pq3d_ens_create(void** ens, long long n, int B, void* const* d_mats_or_null, long long ld, int kc);
+int pq3d_ens_destroy(void* ens);
+int pq3d_ens_matrix_ptrs(void* ens, void** d_ptrs6, long long* ld);
+int pq3d_ens_set_field(void* ens, int field, const double* host_Bn);
+int pq3d_ens_get_field(void* ens, int field, double* host_Bn);
+int pq3d_ens_set_params(void* ens, double V0, double Vpl, double eta);
+/* per member: d5 = time, next dt, last h, last dtnext, 0; i4 = rejections,
+ * status, active, accepted steps */
+int pq3d_ens_members(void* ens, double* d5, long long* i4);
+int pq3d_ens_set_members(void* ens, const double* dttry_or_null, const int* active_or_null);
+int pq3d_ens_gemm(void* ens, int reps, float* ms_per_launch);
+int pq3d_ens_gemm_result(void* ens, double* host_3Bn);
+int pq3d_ens_run(void* ens, long long nsteps, int graph_steps, int* status);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1,0 +1,101 @@
+// FP64 operation count of the Green's-function assembly, per source-receiver
+// pair, from the kernel's own source (td_field.hpp) compiled on the host with a
+// counting scalar type.  Convention: add, sub, mul, div, sqrt, log, atan,
+// atan2 = 1 each (a fused multiply-add is a mul plus an add = 2); comparisons
+// and selects are free.  Transcendentals are counted as 1 although the device
+// evaluates them with many instructions, so the count is a lower bound on the
+// work the hardware does.
+//
+// Usage: flopcount [nx] [samples]  -> prints the mean count per pair for the
+// planar nx x nx mesh of a 40 km x 40 km vertical fault (strike and dip slip,
+// half space, traction projection included), as assembled by gf_assemble_kernel.
+#include <cstdio>
+#include <cstdlib>
+#include <random>
+#include <vector>
+
+static long long g_flops = 0;
+
+struct Cnt {
+    double v;
+    Cnt() : v(0) {}
+    Cnt(double x) : v(x) {}
+};
+static inline Cnt operator+(Cnt a, Cnt b) { ++g_flops; return Cnt(a.v + b.v); }
+static inline Cnt operator-(Cnt a, Cnt b) { ++g_flops; return Cnt(a.v - b.v); }
+static inline Cnt operator*(Cnt a, Cnt b) { ++g_flops; return Cnt(a.v * b.v); }
+static inline Cnt operator/(Cnt a, Cnt b) { ++g_flops; return Cnt(a.v / b.v); }
+static inline Cnt operator-(Cnt a) { return Cnt(-a.v); }
+static inline Cnt& operator+=(Cnt& a, Cnt b) { a = a + b; return a; }
+static inline Cnt& operator-=(Cnt& a, Cnt b) { a = a - b; return a; }
+static inline bool operator<(Cnt a, Cnt b) { return a.v < b.v; }
+static inline bool operator>(Cnt a, Cnt b) { return a.v > b.v; }
+static inline bool operator<=(Cnt a, Cnt b) { return a.v <= b.v; }
+static inline bool operator>=(Cnt a, Cnt b) { return a.v >= b.v; }
+static inline bool operator==(Cnt a, Cnt b) { return a.v == b.v; }
+static inline bool operator!=(Cnt a, Cnt b) { return a.v != b.v; }
+
+#include <cmath>
+// found by argument-dependent lookup from the templates in td_field.hpp
+inline Cnt rsqrt_(Cnt a) { ++g_flops; return Cnt(std::sqrt(a.v)); }
+inline Cnt rlog_(Cnt a) { ++g_flops; return Cnt(std::log(a.v)); }
+inline Cnt ratan_(Cnt a) { ++g_flops; return Cnt(std::atan(a.v)); }
+inline Cnt ratan2_(Cnt a, Cnt b) { ++g_flops; return Cnt(std::atan2(a.v, b.v)); }
+inline Cnt rrcp_(Cnt a) { ++g_flops; return Cnt(1.0 / a.v); }
+
+#include "td_field.hpp"
+
+using namespace pq3d;
+
+int main(int argc, char** argv) {
+    const int nx = argc > 1 ? atoi(argv[1]) : 100;
+    const int samples = argc > 2 ? atoi(argv[2]) : 4000;
+    const double L = 40e3, D = 40e3, mu = 32038120320.0, lam = 32038120320.0;
+    const double nu = 1.0 / (1.0 + lam / mu) / 2.0;
+    // planar vertical fault y = 0 (same ordering as pyquake3d_amd.synth.planar_fault_mesh)
+    auto node = [&](int i, int k, double* p) {
+        p[0] = -0.5 * L + L * i / nx;
+        p[1] = 0.0;
+        p[2] = -D + D * k / nx;
+    };
+    std::vector<double> tri;  // 9 per triangle
+    for (int i = 0; i < nx; ++i)
+        for (int k = 0; k < nx; ++k) {
+            double a[3], b[3], c[3], d[3];
+            node(i, k, a);
+            node(i + 1, k, b);
+            node(i + 1, k + 1, c);
+            node(i, k + 1, d);
+            const double* t1[3] = {a, c, b};
+            const double* t2[3] = {a, d, c};
+            for (auto t : {t1, t2})
+                for (int v = 0; v < 3; ++v)
+                    for (int q = 0; q < 3; ++q) tri.push_back(t[v][q]);
+        }
+    const long long N = (long long)tri.size() / 9;
+    std::mt19937_64 rng(12345);
+    std::uniform_int_distribution<long long> pick(0, N - 1);
+    const double bT[2][3] = {{0.0, 1.0, 0.0}, {0.0, 0.0, 1.0}};
+    long long total = 0;
+    for (int s = 0; s < samples; ++s) {
+        const long long r = pick(rng), j = pick(rng);
+        const double* R = &tri[9 * r];
+        double X = (R[0] + R[3] + R[6]) / 3, Y = (R[1] + R[4] + R[7]) / 3, Z = (R[2] + R[5] + R[8]) / 3;
+        SrcGeom<2> g;
+        src_setup<2>(&tri[9 * j], &tri[9 * j + 3], &tri[9 * j + 6], bT, g);
+        // receiver frame of a y = 0 plane with +y normal: e1 = (1,0,0), e2 = (0,0,-1), n = (0,1,0)
+        const Cnt ev[9] = {1, 0, 0, 0, 0, -1, 0, 1, 0};
+        g_flops = 0;
+        Cnt G[2][9];
+        td_pair_grad<Cnt, 2, false>(g, Cnt(X), Cnt(Y), Cnt(Z), Cnt(nu), true, G);
+        for (int b = 0; b < 2; ++b) {
+            Cnt eps[6], sig[6], t[3];
+            grad_to_strain_stress<Cnt>(G[b], Cnt(mu), Cnt(lam), eps, sig);
+            stress_to_traction<Cnt>(sig, ev, t);
+        }
+        total += g_flops;
+    }
+    printf("%.1f flops_per_pair (mean of %d pairs, %dx%d planar mesh, half space, strike+dip, "
+           "add/sub/mul/div/sqrt/log/atan = 1)\n", (double)total / samples, samples, nx, nx);
+    return 0;
+}

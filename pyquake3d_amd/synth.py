@@ -76,7 +76,8 @@ def patch_distance_km(xg_m, half_len_km=15.0, zc_km=-20.0, half_h_km=7.5):
     return np.sqrt(dx * dx + dz * dz)
 
 
-def friction_params(xg_m, dc_sigma=0.0, seed=0, a_vw=0.004, a_vs=0.04, b=0.03, dc=0.14, sigma_n=SIGMA_N):
+def friction_params(xg_m, dc_sigma=0.0, seed=0, a_vw=0.004, a_vs=0.04, b=0.03, dc=0.14, sigma_n=SIGMA_N,
+                    rake=0.0):
     """Per-element (rake, a, b, Dc, f0, Tt, Tn, V, shear_loading, normal_loading)
     in the column order of the reference's init file."""
     n = len(xg_m)
@@ -90,7 +91,7 @@ def friction_params(xg_m, dc_sigma=0.0, seed=0, a_vw=0.004, a_vs=0.04, b=0.03, d
     sn = np.broadcast_to(np.asarray(sigma_n, dtype=np.float64), (n,)).copy()
     V = np.full(n, VPL)
     tt = sn * a * np.arcsinh(V / (2 * V0) * np.exp((F0 + bb * np.log(V0 / VPL)) / a))
-    return np.stack([np.zeros(n), a, bb, dcv, np.full(n, F0), tt, sn, V, np.zeros(n), np.zeros(n)], axis=1)
+    return np.stack([np.full(n, rake), a, bb, dcv, np.full(n, F0), tt, sn, V, np.zeros(n), np.zeros(n)], axis=1)
 
 
 def member_params(xg_m, member, base_seed=1000):

@@ -1,7 +1,12 @@
 """GPU parity of the dense stepping path against the NumPy FP64 oracle.
 
 Both sides start from the same state and use the same FP64 Green's functions
-(assembled on the GPU, downloaded for the oracle).  Bars:
+(assembled on the GPU, downloaded for the oracle).  The step-by-step cases use
+a mixed rake (0.5 rad): with pure strike slip the dip traction stays at
+rounding-noise size, and the reference's relative error estimate of that
+component is then set by summation order, so any two correct implementations
+(the reference's own NumPy and CuPy paths included) part ways at the first
+borderline accept/reject decision.  Bars:
 * step sizes h, dtnext: relative 1e-9 over the first 120 steps;
 * state vectors: relative to their max magnitude, 1e-9 over those steps;
 * traction matvec: FP64 storage 1e-12 of sum|a_ij u_j| per row, FP32 storage
@@ -23,12 +28,12 @@ CS = 3464.0
 V0, VPL = 1e-6, 1e-9
 
 
-def _case(nx, nz, length, depth, dc_sigma=0.2, seed=3):
+def _case(nx, nz, length, depth, dc_sigma=0.2, seed=3, rake=0.5):
     from pyquake3d_amd import readmsh, synth, greenfunc
     nodes_km, tri = synth.planar_fault_mesh(nx, nz, length, depth)
     nodes = nodes_km * 1e3
     ev, xg = readmsh.get_eleVec(nodes, tri, False)
-    cols = synth.friction_params(xg, dc_sigma=dc_sigma, seed=seed)
+    cols = synth.friction_params(xg, dc_sigma=dc_sigma, seed=seed, rake=rake)
     P = greenfunc.source_vertices(nodes, tri)
     return nodes, tri, ev, xg, cols, P
 
