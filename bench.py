@@ -1,24 +1,27 @@
 """Benchmark of the PyQuake3D hot path on one MI355X.
 
-Workload (N = 1): planar vertical strike-slip fault, 40 km x 40 km, 100 x 100
-squares split into 20,000 triangular dislocations, half space; aging-law
-rate-and-state friction (b = 0.03, Dc = 0.14 m with a seeded lognormal
-perturbation of sigma 0.2, a = 0.004 in a central 30 x 15 km patch rising to
-0.04 outside), sigma_n = 25 MPa, Vpl = V_init = 1e-9 m/s, state at steady state.
+Mesh (all legs): planar vertical strike-slip fault, 40 km x 40 km, 100 x 100
+squares split into 20,000 triangular dislocations, half space.
 
-* Green's functions: FP64 triangular-dislocation assembly on the GPU (6 dense
-  20k x 20k traction matrices), stored in FP32; reported as pairs/s and FP64
-  FLOP/s.
-* Stepping: adaptive Dormand-Prince RK45 with the reference's accept/reject
-  rule; one *step* = one accepted step of the whole simulation (one member).
-  value = accepted member-steps per second over K timed steps, device-resident,
-  no host synchronisation inside the timed region.
-* roofline: the traction matvec (dominant kernel): algorithmic bytes per launch
-  (six 20k x 20k FP32 matrices + the slip-rate vectors) / its average launch
-  time from HIP events; peak 8 TB/s.
+Legs
+----
+* Green's functions: FP64 triangular-dislocation assembly on the GPU (six dense
+  20k x 20k traction matrices), stored in FP32; pairs/s and FP64 FLOP/s.
+* value -- batched ensemble: 64 members with seeded friction parameters
+  (a-b ~ U[-0.006,-0.002] on the patch, Dc = 0.14 exp(0.3 N), sigma_n ~
+  U[20,30] MPa) share the FP32 matrices; the traction rates of all members
+  come from one tall-skinny GEMM on MFMA per global step, and every member
+  takes one accepted adaptive RK45 step with its own step size.
+  value = accepted member-steps per second (64 x global steps / time).
+* plain: one simulation (aging law, Dc lognormal sigma 0.2, V = Vpl = 1e-9
+  m/s, steady state) with FP32 and with FP64 matrix storage; member-steps/s.
+* roofline: the ensemble GEMM (dominant kernel of `value`), 2*6*N^2*64 flops
+  per launch over its mean launch time from HIP events, against the 157.3
+  TFLOP/s FP32 MFMA peak; the plain FP32 matvec against 8 TB/s HBM is reported
+  beside it.
 * cpu_baseline: the NumPy FP64 restatement of the reference's dense step
-  (oracle/qdsim_oracle.py, six np.dot per step, BLAS on all host cores) on the
-  same 20k matrices, timed on a bounded number of steps.
+  (oracle/qdsim_oracle.py: six np.dot per step, BLAS on all host cores) on the
+  same 20k FP64 matrices, over a bounded number of steps.
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W]
 """
@@ -39,6 +42,12 @@ CS = 3464.0
 V0, VPL = 1e-6, 1e-9
 HBM_PEAK_GBS = 8000.0
 FP64_PEAK_TFLOPS = 78.6
+FP32_MFMA_PEAK_TFLOPS = 157.3
+MEMBERS = 64
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
 
 
 def _cpu_model():
@@ -78,7 +87,7 @@ def init_state(sim, cols):
     sim.set_params(V0, VPL, MU / (2 * CS), False, True)
 
 
-def time_steps(sim, steps, warmup, graph_steps):
+def time_plain(sim, steps, warmup, graph_steps):
     from pyquake3d_amd import _lib
     sim.step(1e-3)
     if warmup > 1:
@@ -94,9 +103,23 @@ def time_steps(sim, steps, warmup, graph_steps):
     return ms
 
 
+def time_ensemble(ens, steps, warmup, graph_steps):
+    from pyquake3d_amd import _lib
+    if warmup > 0:
+        assert ens.run(warmup, graph_steps) == 0
+    t = _lib.Timer()
+    _lib.lib().pq3d_synchronize()
+    t.start()
+    st = ens.run(steps, graph_steps)
+    t.stop()
+    ms = t.ms()
+    assert st == 0, f"ensemble failed with status {st}"
+    return ms
+
+
 def assembly_flops_per_pair():
     """FP64 operations per source-receiver pair (both slip directions) from the
-    counting build of the kernel's own source (pyquake3d_amd/csrc/flopcount)."""
+    counting build of the kernel's own source (pyquake3d_amd/csrc/flopcount.cpp)."""
     p = os.path.join(ROOT, "pyquake3d_amd", "csrc", "flops_per_pair.txt")
     try:
         with open(p) as f:
@@ -105,43 +128,22 @@ def assembly_flops_per_pair():
         return None
 
 
-def cpu_baseline_step(case, sim64_mats, budget_s=15.0, max_steps=40):
-    """NumPy FP64 restatement of the dense step on the host (oracle)."""
+def cpu_baseline_step(cols, mats64, budget_s=15.0, max_steps=40):
+    """NumPy FP64 restatement of the dense step on the host (the oracle)."""
     from oracle import qdsim_oracle as Q
     threads = len(os.sched_getaffinity(0))
-    orc = Q.from_init_columns(sim64_mats, case[4], V0, VPL, MU, CS)
-    dt = 1e-3
-    orc.step(dt)  # untimed first step
+    orc = Q.from_init_columns(mats64, cols, V0, VPL, MU, CS)
+    orc.step(1e-3)  # untimed first step
     dt = 1.5e-3
     t0 = time.perf_counter()
     n = 0
     while n < max_steps and time.perf_counter() - t0 < budget_s:
-        h, dt = orc.step(dt)
+        _, dt = orc.step(dt)
         n += 1
     el = time.perf_counter() - t0
     return {"value": n / el, "unit": "member-steps/s", "cores": threads, "kind": "port",
-            "sample": f"{n} steps of the NumPy FP64 restatement of simu_forward (6 np.dot per step) on the "
+            "sample": f"{n} steps of the NumPy FP64 restatement of simu_forward (six np.dot per step) on the "
                       f"same 20,000-element FP64 matrices, BLAS threads = {threads}, CPU {_cpu_model()}"}
-
-
-def cpu_baseline_assembly(case, budget_s=10.0):
-    """Reference C++ TDstressFS_C (built from source into oracle/_ref) over
-    sampled source columns, one process per host core."""
-    from oracle import refbind
-    nodes, tri, ev, xg, cols, P = case
-    if not refbind.available():
-        return None
-    import multiprocessing as mp
-    threads = len(os.sched_getaffinity(0))
-    ncol = 2 * threads
-    t0 = time.perf_counter()
-    with mp.get_context("fork").Pool(threads) as pool:
-        pool.map(_ref_column, [(P[0][j], P[1][j], P[2][j], xg) for j in range(ncol)])
-    el = time.perf_counter() - t0
-    pairs = ncol * len(xg)
-    return {"value": pairs / el, "unit": "pairs/s", "cores": threads, "kind": "reference",
-            "sample": f"{ncol} source columns x {len(xg)} receivers through the reference TDstressFS_C "
-                      f"(strike and dip calls), {threads} processes"}
 
 
 def _ref_column(args):
@@ -152,91 +154,122 @@ def _ref_column(args):
     return 0
 
 
+def cpu_baseline_assembly(P, xg):
+    """The reference's own TDstressFS_C (built from source into oracle/_ref)
+    over sampled source columns, one process per host core."""
+    from oracle import refbind
+    if not refbind.available():
+        return None
+    import multiprocessing as mp
+    threads = len(os.sched_getaffinity(0))
+    ncol = threads
+    t0 = time.perf_counter()
+    with mp.get_context("fork").Pool(threads) as pool:
+        pool.map(_ref_column, [(P[0][j], P[1][j], P[2][j], xg) for j in range(ncol)])
+    el = time.perf_counter() - t0
+    return {"value": ncol * len(xg) / el, "unit": "pairs/s", "cores": threads, "kind": "reference",
+            "sample": f"{ncol} source columns x {len(xg)} receivers through the reference TDstressFS_C "
+                      f"(strike and dip calls), {threads} processes"}
+
+
+def _pmc_traffic(name):
+    """HBM bytes per launch from a committed rocprofv3 --pmc summary, or None."""
+    p = os.path.join(ROOT, "profiles", name)
+    try:
+        with open(p) as f:
+            return float(json.load(f)["hbm_bytes_per_launch"])
+    except (OSError, ValueError, KeyError):
+        return None
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=2000)
-    ap.add_argument("--warmup", type=int, default=20)
-    ap.add_argument("--graph-steps", type=int, default=16)
+    ap.add_argument("--steps", type=int, default=500, help="timed ensemble global steps")
+    ap.add_argument("--warmup", type=int, default=10)
+    ap.add_argument("--plain-steps", type=int, default=2000)
+    ap.add_argument("--graph-steps", type=int, default=8)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--skip-fp64", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
-        print("bench.py: this build runs on one GPU only; multi-GPU sharding is described in DESIGN.md",
+        print("bench.py: this build runs on one GPU only; how the path would shard is described in DESIGN.md",
               file=sys.stderr)
         sys.exit(2)
 
-    from pyquake3d_amd import _lib
+    from pyquake3d_amd import _lib, synth
     from pyquake3d_amd.QDsim_gpu import DenseSim
+    from pyquake3d_amd.ensemble import Ensemble, init_members
 
-    def log(msg):
-        print(msg, file=sys.stderr, flush=True)
-
-    case = build_case()
-    nodes, tri, ev, xg, cols, P = case
+    nodes, tri, ev, xg, cols, P = build_case()
     N = len(tri)
     log(f"[bench] N={N} elements")
 
-    # --- Green's function assembly (FP64 arithmetic, FP32 storage)
+    # --- Green's functions: FP64 arithmetic, FP32 storage
     sim = DenseSim(N, _lib.MAT_F32)
     t = _lib.Timer()
-    sim.assemble(*P, xg, ev, MU, LAM, True)   # first call includes code load
+    sim.assemble(*P, xg, ev, MU, LAM, True)   # first call loads the code object
     t.start()
     sim.assemble(*P, xg, ev, MU, LAM, True)
     t.stop()
     asm_ms = t.ms()
     pairs = float(N) * N
-    fpp = assembly_flops_per_pair()
     gf = {"pairs_per_s": pairs / (asm_ms * 1e-3), "ms": asm_ms, "pairs": pairs,
-          "note": "pairs = receiver x source; each pair evaluates strike and dip slip"}
+          "note": "pair = receiver centroid x source TD, strike and dip slip, half space, traction projection"}
+    fpp = assembly_flops_per_pair()
     if fpp:
         tf = pairs * fpp / (asm_ms * 1e-3) / 1e12
         gf.update({"fp64_flops_per_pair": fpp, "fp64_tflops": tf, "fp64_peak_tflops": FP64_PEAK_TFLOPS,
                    "frac_fp64_peak": tf / FP64_PEAK_TFLOPS})
     log(f"[bench] assembly {asm_ms:.1f} ms, {gf['pairs_per_s']:.3e} pairs/s")
 
-    # --- plain single-simulation stepping, FP32 storage (the measured value)
+    # --- plain single simulation, FP32 storage
     init_state(sim, cols)
     mv_ms = sim.time_matvec(20)
-    step_ms = time_steps(sim, args.steps, args.warmup, args.graph_steps)
-    mv_ms_after = sim.time_matvec(20)
-    st = sim.state()
-    value = args.steps / (step_ms * 1e-3)
-    log(f"[bench] {args.steps} steps in {step_ms:.1f} ms -> {value:.1f} member-steps/s; "
-        f"matvec {mv_ms:.3f}/{mv_ms_after:.3f} ms; sim time {st['time']:.3e} s")
-    mv_avg = 0.5 * (mv_ms + mv_ms_after)
-    alg_bytes = 6.0 * N * N * 4 + 4.0 * N * 8 + 3.0 * sim_chunks(sim) * N * 8
-    achieved = alg_bytes / (mv_avg * 1e-3) / 1e9
-    roofline = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": achieved / HBM_PEAK_GBS, "traffic": _pmc_traffic(),
-                "kernel": "matvec_kernel<float> (six FP32 traction matrices, FP64 accumulation)",
-                "alg_bytes_per_launch": alg_bytes, "avg_launch_ms": mv_avg}
+    p32_ms = time_plain(sim, args.plain_steps, args.warmup, args.graph_steps)
+    plain32 = args.plain_steps / (p32_ms * 1e-3)
+    mv_alg = 6.0 * N * N * 4 + 4.0 * N * 8 + 3.0 * ((N + 255) // 256) * N * 8
+    mv_gbs = mv_alg / (mv_ms * 1e-3) / 1e9
+    log(f"[bench] plain FP32: {plain32:.1f} member-steps/s; matvec {mv_ms:.3f} ms = {mv_gbs:.0f} GB/s")
 
-    # --- plain FP64 path (denominator of the batched-vs-plain ratio)
-    extra = {}
-    sim64_mats = None
-    if not args.skip_fp64:
-        sim.close()
-        s64 = DenseSim(N, _lib.MAT_F64)
-        s64.assemble(*P, xg, ev, MU, LAM, True)
-        init_state(s64, cols)
-        k64 = max(50, args.steps // 8)
-        ms64 = time_steps(s64, k64, args.warmup, args.graph_steps)
-        extra["plain_fp64_member_steps_per_s"] = k64 / (ms64 * 1e-3)
-        extra["plain_fp64_matvec_ms"] = s64.time_matvec(10)
-        log(f"[bench] FP64 storage: {extra['plain_fp64_member_steps_per_s']:.1f} member-steps/s")
-        if not args.no_cpu_baseline:
-            sim64_mats = [s64.download_matrix(k) for k in range(6)]
-        s64.close()
+    # --- batched ensemble on the same FP32 matrices (the measured value)
+    mcols = np.stack([synth.member_params(xg, m) for m in range(MEMBERS)])
+    ens = Ensemble(N, MEMBERS, matrices_from=sim)
+    init_members(ens, mcols, V0, VPL, MU, CS)
+    ens_ms = time_ensemble(ens, args.steps, args.warmup, args.graph_steps)
+    value = MEMBERS * args.steps / (ens_ms * 1e-3)
+    gemm_ms = ens.gemm(reps=10)
+    mem = ens.members()
+    gemm_flops = 2.0 * 6 * N * N * MEMBERS
+    gemm_tf = gemm_flops / (gemm_ms * 1e-3) / 1e12
+    log(f"[bench] ensemble: {value:.1f} member-steps/s ({args.steps} global steps in {ens_ms:.1f} ms); "
+        f"GEMM {gemm_ms:.3f} ms = {gemm_tf:.1f} TFLOP/s")
+    ens_info = {"members": MEMBERS, "global_steps": args.steps, "ms_per_global_step": ens_ms / args.steps,
+                "gemm_ms": gemm_ms, "member_sim_time_s_min": float(mem["time"].min()),
+                "member_sim_time_s_max": float(mem["time"].max()), "failed_members": int((mem["status"] != 0).sum())}
+    ens.close()
+    sim.close()
 
+    # --- plain single simulation, FP64 storage (denominator of batched vs plain)
+    s64 = DenseSim(N, _lib.MAT_F64)
+    s64.assemble(*P, xg, ev, MU, LAM, True)
+    init_state(s64, cols)
+    k64 = max(100, args.plain_steps // 4)
+    p64_ms = time_plain(s64, k64, args.warmup, args.graph_steps)
+    plain64 = k64 / (p64_ms * 1e-3)
+    mv64_ms = s64.time_matvec(10)
+    log(f"[bench] plain FP64: {plain64:.1f} member-steps/s; matvec {mv64_ms:.3f} ms")
     cpu = None
-    if not args.no_cpu_baseline and sim64_mats is not None:
-        cpu = cpu_baseline_step(case, sim64_mats)
-        del sim64_mats
+    if not args.no_cpu_baseline:
+        mats64 = [s64.download_matrix(k) for k in range(6)]
+        s64.close()
+        cpu = cpu_baseline_step(cols, mats64)
+        del mats64
         log(f"[bench] CPU baseline {cpu['value']:.3f} member-steps/s on {cpu['cores']} threads")
-        asm_cpu = cpu_baseline_assembly(case)
+        asm_cpu = cpu_baseline_assembly(P, xg)
         if asm_cpu:
             gf["cpu_baseline"] = asm_cpu
+    else:
+        s64.close()
 
     out = {
         "metric": METRIC,
@@ -245,42 +278,36 @@ def main():
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": step_ms / args.steps,
+        "ms_per_step": ens_ms / args.steps,
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f64",
         "data": "synthetic",
-        "config": {"workload": "planar vertical fault 40x40 km, 100x100 grid -> 20,000 TDs, half space; "
-                               "FP64 GF assembly, FP32 GF storage (FP64 accumulation); aging-law RSF, "
-                               "RK45; plain single simulation (1 member)",
-                   "n_elements": N, "members": 1, "gf_storage": "fp32", "state_dtype": "f64",
-                   "matrices_streamed_per_step": 6, "parallelism": "single GPU"},
-        "roofline": roofline,
+        "config": {"workload": "batched ensemble: 64 members (seeded a-b, Dc, sigma_n) on a planar vertical fault, "
+                               "40x40 km, 100x100 grid -> 20,000 TDs, half space; FP64 GF assembly, FP32 GF storage, "
+                               "GEMM on FP32 MFMA; aging-law RSF with radiation damping, adaptive RK45 per member; "
+                               "a step = one global step (one accepted step of every member)",
+                   "n_elements": N, "members": MEMBERS, "gf_storage": "fp32", "state_dtype": "f64",
+                   "parallelism": "single GPU"},
+        "roofline": {"bound": "mfma", "achieved": gemm_tf, "peak": FP32_MFMA_PEAK_TFLOPS, "unit": "TFLOP/s",
+                     "frac": gemm_tf / FP32_MFMA_PEAK_TFLOPS, "traffic": _pmc_traffic("ens_gemm_20k_traffic.json"),
+                     "kernel": "ens_gemm_kernel (v_mfma_f32_16x16x4f32, 6 x 20k x 20k FP32 by 20k x 64)",
+                     "flops_per_launch": gemm_flops, "avg_launch_ms": gemm_ms},
         "cpu_baseline": cpu,
+        "plain_fp32_member_steps_per_s": plain32,
+        "plain_fp64_member_steps_per_s": plain64,
+        "batched_vs_plain_fp64": value / plain64,
+        "batched_vs_plain_fp32": value / plain32,
+        "plain_roofline": {"bound": "hbm", "achieved": mv_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                           "frac": mv_gbs / HBM_PEAK_GBS, "traffic": _pmc_traffic("matvec_f32_20k_traffic.json"),
+                           "kernel": "matvec_kernel<float> (six FP32 traction matrices, FP64 accumulation)",
+                           "alg_bytes_per_launch": mv_alg, "avg_launch_ms": mv_ms},
+        "ensemble": ens_info,
         "gf_assembly": gf,
         "p99_solve_latency_s": None,
-        "sim_time_s": st["time"],
-        "rejections": None,
     }
-    out.update(extra)
     print(json.dumps(out))
-
-
-def sim_chunks(sim):
-    return (sim.n + 255) // 256
-
-
-def _pmc_traffic():
-    """HBM bytes per matvec launch from a rocprofv3 --pmc summary committed
-    under profiles/ (FETCH_SIZE + WRITE_SIZE, scaled by the copy-kernel
-    calibration recorded there), or None."""
-    p = os.path.join(ROOT, "profiles", "matvec_f32_20k_traffic.json")
-    try:
-        with open(p) as f:
-            return float(json.load(f)["hbm_bytes_per_launch"])
-    except (OSError, ValueError, KeyError):
-        return None
 
 
 if __name__ == "__main__":
