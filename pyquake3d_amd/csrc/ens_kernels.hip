@@ -20,11 +20,13 @@
 //      drop out of the attempt rounds while the others retry (retries reuse
 //      the step's GEMM, as the reference freezes AdotV per step).  Each round:
 //      attempts for all pending (member, element) items -> per-member error
-//      maxima (wave shuffles + one atomic per wave) -> grid barrier -> one
-//      thread per member applies the reference's accept/reject rule -> grid
-//      barrier -> accepted members' items write the post-step state.
+//      maxima (wave shuffles + one atomic per wave) -> ONE grid barrier -> every
+//      workgroup applies the reference's accept/reject rule to the same
+//      reduced errors in its own LDS copy of the member control -> accepted
+//      members' items write the post-step state.
 #include "common.hpp"
 
+#include <cmath>
 #include <cstdint>
 
 namespace pq3d {
@@ -89,17 +91,16 @@ __global__ void ens_prep_u_kernel(const double* __restrict__ V1, const double* _
     U2[t] = u2;
 }
 
-// grid: (ceil(n / 256), nchunk); block 256 = 4 waves, each 64 rows x 64 members.
+// grid: (ceil(n / 64), nchunk); block 64 = one wave owning 64 rows x 64 members.
 // Requires B == 64 and ld % 4 == 0 (rows padded, zero-filled).
-__global__ __launch_bounds__(256) void ens_gemm_kernel(const float* __restrict__ A1s, const float* __restrict__ A2s,
+__global__ __launch_bounds__(64) void ens_gemm_kernel(const float* __restrict__ A1s, const float* __restrict__ A2s,
                                                        const float* __restrict__ Bs, const float* __restrict__ A1d,
                                                        const float* __restrict__ A2d, const float* __restrict__ Bd,
                                                        long long ld, long long n, int kc,
                                                        const float* __restrict__ U1, const float* __restrict__ U2,
                                                        float* __restrict__ partial, long long npad) {
     const int lane = threadIdx.x & 63;
-    const int wave = threadIdx.x >> 6;
-    const long long r0 = (long long)blockIdx.x * 256 + wave * 64;  // this wave's first row
+    const long long r0 = (long long)blockIdx.x * 64;  // this wave's first row
     const long long k0 = (long long)blockIdx.y * kc;
     long long k1 = k0 + kc;
     if (k1 > n) k1 = n;
@@ -296,16 +297,23 @@ __global__ __launch_bounds__(256) void ens_rk_kernel(EnsArgs A) {
     auto F = [&](int field, long long idx) -> double& { return f[(size_t)field * items + idx]; };
     const int par = (int)(__hip_atomic_load(&A.ctrl->gsteps, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) & 1);
     unsigned long long* slots = A.slot + (size_t)par * 24 * B * 2;
-    // reset the other parity's slots and this step's member flags
-    for (long long t = t0; t < 24 * B * 2; t += stride) A.slot[(size_t)(par ^ 1) * 24 * B * 2 + t] = 0ull;
-    if (t0 < B) {
-        MemberCtrl& c = A.mc[t0];
-        c.done = c.active ? 0 : 1;
-        c.h = c.dttry;
-        c.nrjct = 0;
-        c.vmax = 0.0;
+    // Every workgroup keeps its own copy of the member control in LDS and
+    // applies the same deterministic accept/reject rule to the same reduced
+    // errors, so one grid barrier per attempt round suffices.
+    // s_st: 0 pending, 1 inactive, 2 accepted in this round, 3 failed, 4 accepted earlier
+    __shared__ double s_h[64], s_dtnext[64];
+    __shared__ int s_st[64], s_nrjct[64];
+    __shared__ int s_any;
+    if (threadIdx.x < B) {
+        const MemberCtrl c = A.mc[threadIdx.x];
+        s_h[threadIdx.x] = c.dttry;
+        s_dtnext[threadIdx.x] = c.dttry;
+        s_st[threadIdx.x] = c.active ? 0 : 1;
+        s_nrjct[threadIdx.x] = 0;
     }
-    // frozen traction rates (sum of the GEMM's split-K partials, fixed order)
+    for (long long t = t0; t < 24 * B * 2; t += stride) A.slot[(size_t)(par ^ 1) * 24 * B * 2 + t] = 0ull;
+    // frozen traction rates (sum of the GEMM's split-K partials, fixed order);
+    // each item is produced and later consumed by the same thread
     for (long long t = t0; t < items; t += stride) {
         const int m = (int)(t / n);
         const long long e = t - (long long)m * n;
@@ -319,16 +327,15 @@ __global__ __launch_bounds__(256) void ens_rk_kernel(EnsArgs A) {
         F(E_ADOTV2, t) = s1;
         F(E_DSIGMADT, t) = s2 + F(E_NORMAL_LOADING, t);
     }
-    if (!ens_barrier(A.ctrl, (unsigned)A.nblocks, A.spin_limit)) return;
+    __syncthreads();
 
     for (int att = 0; att < 22; ++att) {
-        // attempts of every pending member
         for (long long base = (long long)blockIdx.x * blockDim.x; base < items; base += stride) {
             const long long t = base + threadIdx.x;
             const bool in = t < items;
             const long long tt = in ? t : items - 1;
             const int m = (int)(tt / n);
-            const bool pend = in && !A.mc[m].done;
+            const bool pend = in && s_st[m] == 0;
             unsigned long long r1b = 0ull, r2b = 0ull;
             if (pend) {
                 EElem e;
@@ -338,7 +345,7 @@ __global__ __launch_bounds__(256) void ens_rk_kernel(EnsArgs A) {
                 e.V = F(E_SLIPV, tt);
                 const double y[4] = {F(E_TNO, tt), F(E_TT1O, tt), F(E_TT2O, tt), F(E_STATE, tt)};
                 double y5[4], y4[4];
-                ens_dp45(e, A.V0, A.eta, y, A.mc[m].h, y5, y4);
+                ens_dp45(e, A.V0, A.eta, y, s_h[m], y5, y4);
                 F(E_Y5_0, tt) = y5[0]; F(E_Y5_1, tt) = y5[1]; F(E_Y5_2, tt) = y5[2]; F(E_Y5_3, tt) = y5[3];
                 const double r1 = fabs(fabs(y4[3] - y5[3]) / y4[3]);
                 const double rt1 = fabs(fabs(y4[1] - y5[1]) / y4[1]);
@@ -350,42 +357,38 @@ __global__ __launch_bounds__(256) void ens_rk_kernel(EnsArgs A) {
             member_max_atomic(m, pend, r1b, r2b, slots + (size_t)att * B * 2);
         }
         if (!ens_barrier(A.ctrl, (unsigned)A.nblocks, A.spin_limit)) return;
-        // per-member decision (QDsim_gpu.py:959-985), one thread per member
-        if (t0 < B) {
-            MemberCtrl& c = A.mc[t0];
-            if (!c.done) {
-                const double e1 = eflt(slots[((size_t)att * B + t0) * 2 + 0]) + 1e-10;
-                const double e2 = eflt(slots[((size_t)att * B + t0) * 2 + 1]) + 1e-10;
-                const double c1 = e1 / 1e-4, c2 = e2 / 1e-4;
-                const double h = c.h;
-                const double hn1 = h * 0.9 * pow(1e-4 / e1, 0.2);
-                const double hn2 = h * 0.9 * pow(1e-4 / e2, 0.2);
-                if (epy_max(c1, c2) < 1.0 && !(c1 != c1 || c2 != c2)) {
-                    c.dtnext = epy_min(1.5 * h, epy_min(hn1, hn2));
-                    c.done = 2;  // accepted in this round: post-update pending
-                } else {
-                    c.nrjct += 1;
-                    const double hn = epy_max(0.5 * h, epy_min(hn1, hn2));
-                    c.h = hn;
-                    if (hn < 1.e-15 || c.nrjct > 20) {
-                        c.status = 1;
-                        c.done = 3;  // failed: member masked from now on
-                        c.active = 0;
-                    }
-                }
+        // per-member decision (QDsim_gpu.py:959-985), replicated in every workgroup
+        if (threadIdx.x < B && s_st[threadIdx.x] == 0) {
+            const int m = threadIdx.x;
+            const double e1 = eflt(__hip_atomic_load(slots + ((size_t)att * B + m) * 2 + 0, __ATOMIC_RELAXED,
+                                                     __HIP_MEMORY_SCOPE_AGENT)) + 1e-10;
+            const double e2 = eflt(__hip_atomic_load(slots + ((size_t)att * B + m) * 2 + 1, __ATOMIC_RELAXED,
+                                                     __HIP_MEMORY_SCOPE_AGENT)) + 1e-10;
+            const double c1 = e1 / 1e-4, c2 = e2 / 1e-4;
+            const double h = s_h[m];
+            const double hn1 = h * 0.9 * pow(1e-4 / e1, 0.2);
+            const double hn2 = h * 0.9 * pow(1e-4 / e2, 0.2);
+            if (epy_max(c1, c2) < 1.0 && !(c1 != c1 || c2 != c2)) {
+                s_dtnext[m] = epy_min(1.5 * h, epy_min(hn1, hn2));
+                s_st[m] = 2;
+            } else {
+                s_nrjct[m] += 1;
+                const double hn = epy_max(0.5 * h, epy_min(hn1, hn2));
+                s_h[m] = hn;
+                if (hn < 1.e-15 || s_nrjct[m] > 20) s_st[m] = 3;
             }
         }
-        if (!ens_barrier(A.ctrl, (unsigned)A.nblocks, A.spin_limit)) return;
+        __syncthreads();
         // post-step update of members accepted in this round (QDsim_gpu.py:987-1022)
-        bool any_pending = false;
         for (long long t = t0; t < items; t += stride) {
             const int m = (int)(t / n);
-            if (A.mc[m].done != 2) continue;
-            const double h = A.mc[m].h;
+            if (s_st[m] != 2) continue;
+            const double h = s_h[m];
             const double Tno = F(E_Y5_0, t), Tt1 = F(E_Y5_1, t), Tt2 = F(E_Y5_2, t), st = F(E_Y5_3, t);
             const double a = F(E_A, t);
             F(E_TNO, t) = Tno; F(E_TT1O, t) = Tt1; F(E_TT2O, t) = Tt2; F(E_STATE, t) = st;
-            F(E_TT, t) = sqrt(Tt1 * Tt1 + Tt2 * Tt2);
+            const double Tt = sqrt(Tt1 * Tt1 + Tt2 * Tt2);
+            F(E_TT, t) = Tt;
             const double ex = 2.0 * A.V0 * exp(-st / a);
             const double v1 = ex * sinh(Tt1 / Tno / a), v2 = ex * sinh(Tt2 / Tno / a);
             F(E_SLIPV1, t) = v1; F(E_SLIPV2, t) = v2;
@@ -395,21 +398,34 @@ __global__ __launch_bounds__(256) void ens_rk_kernel(EnsArgs A) {
             F(E_SLIPV, t) = v;
             const double s1 = F(E_SLIP1, t) + v1 * h, s2 = F(E_SLIP2, t) + v2 * h;
             F(E_SLIP1, t) = s1; F(E_SLIP2, t) = s2; F(E_SLIP, t) = sqrt(s1 * s1 + s2 * s2);
-            F(E_FRIC, t) = sqrt(Tt1 * Tt1 + Tt2 * Tt2) / (Tno - F(E_P, t) * 1e-6);
+            F(E_FRIC, t) = Tt / (Tno - F(E_P, t) * 1e-6);
         }
-        if (!ens_barrier(A.ctrl, (unsigned)A.nblocks, A.spin_limit)) return;
-        if (t0 < B) {
-            MemberCtrl& c = A.mc[t0];
-            if (c.done == 2) {
-                c.time = c.time + c.h;
-                c.dttry = c.dtnext;
-                c.steps += 1;
-                c.done = 1;
-            }
+        __syncthreads();
+        if (threadIdx.x < B && s_st[threadIdx.x] == 2) s_st[threadIdx.x] = 4;
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            int any = 0;
+            for (int m = 0; m < B; ++m) any |= (s_st[m] == 0);
+            s_any = any;
         }
-        for (int m = 0; m < B; ++m) any_pending |= (A.mc[m].done == 0);
-        if (!ens_barrier(A.ctrl, (unsigned)A.nblocks, A.spin_limit)) return;
-        if (!any_pending) break;
+        __syncthreads();
+        if (!s_any) break;
+    }
+    if (blockIdx.x == 0 && threadIdx.x < B) {
+        const int m = threadIdx.x;
+        MemberCtrl& c = A.mc[m];
+        c.nrjct = s_nrjct[m];
+        if (s_st[m] == 4) {
+            c.time = c.time + s_h[m];
+            c.h = s_h[m];
+            c.dtnext = s_dtnext[m];
+            c.dttry = s_dtnext[m];
+            c.steps += 1;
+        } else if (s_st[m] == 3 || s_st[m] == 0) {
+            c.h = s_h[m];
+            c.status = 1;
+            c.active = 0;
+        }
     }
     if (t0 == 0) A.ctrl->gsteps += 1;
 }
@@ -467,8 +483,8 @@ static void ens_launch_gemm(Ensemble* e, hipStream_t st) {
     hipLaunchKernelGGL(ens_prep_u_kernel, dim3((unsigned)((items + 255) / 256)), dim3(256), 0, st,
                        f + (size_t)E_SLIPV1 * items, f + (size_t)E_SLIPV2 * items, f + (size_t)E_COSR0 * items,
                        f + (size_t)E_SINR0 * items, e->Vpl, e->n, e->B, e->mc, U1, U2);
-    dim3 grid((unsigned)((e->n + 255) / 256), (unsigned)e->nchunk);
-    hipLaunchKernelGGL(ens_gemm_kernel, grid, dim3(256), 0, st, e->mats[0], e->mats[1], e->mats[2], e->mats[3],
+    dim3 grid((unsigned)((e->n + 63) / 64), (unsigned)e->nchunk);
+    hipLaunchKernelGGL(ens_gemm_kernel, grid, dim3(64), 0, st, e->mats[0], e->mats[1], e->mats[2], e->mats[3],
                        e->mats[4], e->mats[5], e->ld, e->n, e->kc, U1, U2, e->partial, e->npad);
 }
 
@@ -493,7 +509,32 @@ int pq3d_ens_create(void** out, long long n, int B, void* const* d_mats, long lo
     Ensemble* e = new Ensemble();
     e->n = n;
     e->B = B;
-    e->kc = kc > 0 ? (kc + 3) / 4 * 4 : 512;
+    if (kc > 0) {
+        e->kc = (kc + 3) / 4 * 4;
+    } else {
+        // choose the number of column chunks so that the one-wave GEMM tiles
+        // (64 rows x chunk) fill whole waves of the device's concurrent slots,
+        // with a small penalty on the split-K partial traffic
+        int dev = 0, per_cu = 0;
+        PQ_HIP(hipGetDevice(&dev));
+        hipDeviceProp_t p;
+        PQ_HIP(hipGetDeviceProperties(&p, dev));
+        PQ_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, ens_gemm_kernel, 64, 0));
+        const double slots = (double)per_cu * p.multiProcessorCount;
+        const long long tiles = (n + 63) / 64;
+        double best = -1.0;
+        int best_nc = 8;
+        for (int nc = 4; nc <= 32; ++nc) {
+            const double W = (double)tiles * nc;
+            const double waves = std::ceil(W / slots);
+            const double score = W / (waves * slots) - 0.002 * nc;
+            if (score > best) {
+                best = score;
+                best_nc = nc;
+            }
+        }
+        e->kc = (int)(((n + best_nc - 1) / best_nc + 3) / 4 * 4);
+    }
     e->nchunk = (int)((n + e->kc - 1) / e->kc);
     e->npad = (n + 255) / 256 * 256;
     if (d_mats) {

@@ -38,7 +38,7 @@ _lib.register(_declare)
 
 
 class Ensemble:
-    def __init__(self, n, B=64, matrices_from=None, kc=512):
+    def __init__(self, n, B=64, matrices_from=None, kc=0):
         L = _lib.lib()
         h = ctypes.c_void_p()
         if matrices_from is not None:
