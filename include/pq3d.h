@@ -135,24 +135,34 @@ int pq3d_dense_reset_ctrl(void* sim);
  * 5. Ensemble: B = 64 simulations on one mesh (no reference counterpart;
  *    the reference runs independent QDsim_gpu instances)
  *
- * One global step = one accepted RK45 step per active member, each with its
- * own step size; the six FP32 matrices are streamed once per global step by a
- * tall-skinny MFMA GEMM.  Field arrays are B x n (member-major), field
- * indices as in section 4.
+ * One global step = one tall-skinny MFMA GEMM streaming the six FP32 matrices
+ * once for all members, then up to two Dormand-Prince attempt rounds per
+ * active member with its own step size (a member still rejecting continues
+ * at the next global step with the same frozen traction rates).  All work of
+ * an ensemble runs on its own stream.  Field arrays are B x n (member-major),
+ * field indices as in section 4.
  * ---------------------------------------------------------------------- */
 int pq3d_ens_create(void** ens, long long n, int B, void* const* d_mats_or_null, long long ld, int kc);
 int pq3d_ens_destroy(void* ens);
 int pq3d_ens_matrix_ptrs(void* ens, void** d_ptrs6, long long* ld);
 int pq3d_ens_set_field(void* ens, int field, const double* host_Bn);
 int pq3d_ens_get_field(void* ens, int field, double* host_Bn);
+int pq3d_ens_set_member_field(void* ens, int member, int field, const double* host_n);
+int pq3d_ens_get_member_field(void* ens, int member, int field, double* host_n);
 int pq3d_ens_set_params(void* ens, double V0, double Vpl, double eta);
-/* per member: d5 = time, next dt, last h, last dtnext, 0; i4 = rejections,
- * status, active, accepted steps */
-int pq3d_ens_members(void* ens, double* d5, long long* i4);
-int pq3d_ens_set_members(void* ens, const double* dttry_or_null, const int* active_or_null);
+/* per member: d6 = time, next trial step, last accepted step, last dtnext,
+ * t_end, 0; i6 = rejections since the last accepted step, status, active,
+ * finished, accepted steps, global step at which it finished */
+int pq3d_ens_members(void* ens, double* d6, long long* i6);
+/* (re)start a slot: time 0, trial step h0, finish when time >= t_end (0: never) */
+int pq3d_ens_reset_member(void* ens, int member, double h0, double t_end, int active);
 int pq3d_ens_gemm(void* ens, int reps, float* ms_per_launch);
 int pq3d_ens_gemm_result(void* ens, double* host_3Bn);
-int pq3d_ens_run(void* ens, long long nsteps, int graph_steps, int* status);
+/* enqueue nsteps global steps (HIP graph of graph_steps); async != 0 returns
+ * without waiting */
+int pq3d_ens_run(void* ens, long long nsteps, int graph_steps, int async);
+int pq3d_ens_sync(void* ens);
+int pq3d_ens_query(void* ens, int* done);
 
 #ifdef __cplusplus
 }

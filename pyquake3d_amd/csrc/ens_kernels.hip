@@ -1,29 +1,36 @@
-// Ensemble mode: B simulations ("members") with their own friction parameters
-// and state on one shared mesh and one shared set of traction Green's
-// functions.  There is no reference counterpart: the reference runs
-// independent QDsim_gpu instances (main_gpu.py:106-187), each streaming the
-// six matrices once per step.
+// Ensemble mode: B simulations ("members", one per slot) with their own
+// friction parameters and state on one shared mesh and one shared set of
+// traction Green's functions.  There is no reference counterpart: the
+// reference runs independent QDsim_gpu instances (main_gpu.py:106-187), each
+// streaming the six matrices once per step.
 //
-// Per global step:
-//   1. ens_prep_u_kernel:  u = V - Vpl (cos rake0, sin rake0) for every member,
-//      as FP32 [source][member] (members contiguous).
+// Per global step (all launches on the ensemble's own stream, so ensembles of
+// different meshes overlap on the device; no kernel needs co-residency):
+//   1. ens_prep_u_kernel:  u = V - Vpl (cos rake0, sin rake0) of every active
+//      member as FP32 [source][member]; resets the per-step member flags.
 //   2. ens_gemm_kernel:    Y_q = A_q^s U1 + A_q^d U2 for q in {1, 2, n}: one
-//      tall-skinny GEMM (N x 2N by 2N x B) per traction component on the MFMA
-//      units (v_mfma_f32_16x16x4_f32, FP32 in, FP32 accumulate).  A wave owns
+//      tall-skinny GEMM (N x 2N by 2N x 64) per traction component on the MFMA
+//      units (v_mfma_f32_16x16x4_f32, FP32 in, FP32 accumulate).  One wave owns
 //      64 receivers x 64 members x 3 components (192 accumulators); lane l
 //      feeds the four 16-row MFMA subtiles from ONE 16-byte load of the
 //      source-major matrices (row = 4*(l%16) + subtile) and the four 16-member
 //      subtiles from one 16-byte load of u (member = 4*(l%16) + subtile).
 //      Split-K partials per column chunk are summed in a fixed order.
-//   3. ens_rk_kernel:      persistent and masked.  Every member takes exactly
-//      one accepted Dormand-Prince step with its own h; members that accept
-//      drop out of the attempt rounds while the others retry (retries reuse
-//      the step's GEMM, as the reference freezes AdotV per step).  Each round:
-//      attempts for all pending (member, element) items -> per-member error
-//      maxima (wave shuffles + one atomic per wave) -> ONE grid barrier -> every
-//      workgroup applies the reference's accept/reject rule to the same
-//      reduced errors in its own LDS copy of the member control -> accepted
-//      members' items write the post-step state.
+//   3. R rounds of
+//        ens_attempt_kernel: one Dormand-Prince attempt for every (member,
+//          element) item of members still pending in this step; per-member
+//          error maxima by wave shuffles and one atomic per wave;
+//        ens_decide_kernel:  every workgroup applies the reference's
+//          accept/reject rule (QDsim_gpu.py:959-985) to the same reduced
+//          errors; items of members accepted in this round write the
+//          post-step state; workgroup 0 publishes the member control into the
+//          other half of a double buffer (readers of this launch see the old
+//          half).
+//      A member still rejecting after R rounds keeps its state, reduced step
+//      size and rejection count and continues at the next global step, whose
+//      GEMM reproduces the same frozen traction rates (the reference freezes
+//      AdotV per step, QDsim_gpu.py:946-957), so the member's trajectory is
+//      exactly that of the reference's retry loop.
 #include "common.hpp"
 
 #include <cmath>
@@ -39,45 +46,49 @@ enum EField : int {
     E_ADOTV1, E_ADOTV2, E_DSIGMADT, E_COSR0, E_SINR0, E_Y5_0, E_Y5_1, E_Y5_2, E_Y5_3, E_NFIELDS
 };
 
+constexpr int ENS_ROUNDS = 2;  // attempt rounds per global step (even: member control parity repeats)
+
 struct MemberCtrl {
-    double h;          // current attempt's step size
-    double dttry;      // step size to try at the next global step
-    double time;
-    double dtnext;
-    double vmax;
-    int nrjct;
-    int done;          // accepted in this global step
-    int status;        // 0 ok, 1 dt too small / >20 rejections
-    int active;        // masked out when 0
-    long long steps;
+    double h;            // step size of the next attempt
+    double time;         // simulated time
+    double last_h;       // last accepted step
+    double dtnext;       // last accepted step's proposal
+    double t_end;        // finish when time >= t_end (0: never)
+    int nrjct;           // rejections since the last accepted step
+    int accepted;        // accepted in the current global step
+    int status;          // 0 ok, 1 dt too small / >20 rejections
+    int active;          // takes part in global steps
+    int finished;        // reached t_end
+    int pad;
+    long long steps;     // accepted steps
+    long long finish_gstep;
 };
 
 struct EnsCtrl {
-    unsigned bar_count, bar_gen;
-    int status;        // 2: barrier timeout
-    int pad;
-    long long gsteps;
+    long long gsteps;    // global steps started
+    long long pad;
 };
 
 struct EnsArgs {
     double* f;                 // E_NFIELDS x B x n
     const float* partial;      // nchunk x 3 x B x npad
-    MemberCtrl* mc;            // B
-    unsigned long long* slot;  // 2 x 24 x B x 2 (parity, attempt, member, {e1, e2})
+    MemberCtrl* mc;            // 2 x B (double buffer)
+    unsigned long long* slot;  // ENS_ROUNDS x B x 2 error maxima (bits)
     EnsCtrl* ctrl;
     long long n, npad;
-    int B, nchunk, nblocks;
+    int B, nchunk;
     double V0, Vpl, eta;
-    long long spin_limit;
 };
 
 // ---------------------------------------------------------------------------
 __global__ void ens_prep_u_kernel(const double* __restrict__ V1, const double* __restrict__ V2,
                                   const double* __restrict__ c0, const double* __restrict__ s0, double Vpl,
-                                  long long n, int B, const MemberCtrl* __restrict__ mc, float* __restrict__ U1,
-                                  float* __restrict__ U2) {
+                                  long long n, int B, MemberCtrl* __restrict__ mc, EnsCtrl* __restrict__ ctrl,
+                                  float* __restrict__ U1, float* __restrict__ U2) {
     // thread per (source j, member m); writes U[j*B + m]
     const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < B) mc[t].accepted = 0;
+    if (t == 0) ctrl->gsteps += 1;
     if (t >= n * B) return;
     const long long j = t / B;
     const int m = (int)(t - j * B);
@@ -94,11 +105,11 @@ __global__ void ens_prep_u_kernel(const double* __restrict__ V1, const double* _
 // grid: (ceil(n / 64), nchunk); block 64 = one wave owning 64 rows x 64 members.
 // Requires B == 64 and ld % 4 == 0 (rows padded, zero-filled).
 __global__ __launch_bounds__(64) void ens_gemm_kernel(const float* __restrict__ A1s, const float* __restrict__ A2s,
-                                                       const float* __restrict__ Bs, const float* __restrict__ A1d,
-                                                       const float* __restrict__ A2d, const float* __restrict__ Bd,
-                                                       long long ld, long long n, int kc,
-                                                       const float* __restrict__ U1, const float* __restrict__ U2,
-                                                       float* __restrict__ partial, long long npad) {
+                                                      const float* __restrict__ Bs, const float* __restrict__ A1d,
+                                                      const float* __restrict__ A2d, const float* __restrict__ Bd,
+                                                      long long ld, long long n, int kc,
+                                                      const float* __restrict__ U1, const float* __restrict__ U2,
+                                                      float* __restrict__ partial, long long npad) {
     const int lane = threadIdx.x & 63;
     const long long r0 = (long long)blockIdx.x * 64;  // this wave's first row
     const long long k0 = (long long)blockIdx.y * kc;
@@ -146,7 +157,7 @@ __global__ __launch_bounds__(64) void ens_gemm_kernel(const float* __restrict__ 
         }
     }
     // C fragment: lane holds tile rows 4*(lane/16) + v and tile column lane%16.
-    // Tile (i, j) row t  -> physical row r0 + 4*t + i; tile column c -> member 4*c + j.
+    // Tile (i, j) row t -> physical row r0 + 4*t + i; tile column c -> member 4*c + j.
 #pragma unroll
     for (int q = 0; q < 3; ++q)
 #pragma unroll
@@ -155,14 +166,15 @@ __global__ __launch_bounds__(64) void ens_gemm_kernel(const float* __restrict__ 
 #pragma unroll
             for (int v = 0; v < 4; ++v) {
                 const long long prow = r0 + 4 * (4 * lk + v);
-                if (prow < npad) {
-                    float4 o = make_float4(acc[q][0][j][v], acc[q][1][j][v], acc[q][2][j][v], acc[q][3][j][v]);
-                    *reinterpret_cast<float4*>(partial + (((size_t)blockIdx.y * 3 + q) * 64 + member) * npad + prow) = o;
-                }
+                float4 o = make_float4(acc[q][0][j][v], acc[q][1][j][v], acc[q][2][j][v], acc[q][3][j][v]);
+                *reinterpret_cast<float4*>(partial + (((size_t)blockIdx.y * 3 + q) * 64 + member) * npad + prow) = o;
             }
         }
 }
 
+// ---------------------------------------------------------------------------
+// Rate-and-state RHS and the reference's Dormand-Prince attempt
+// (QDsim_gpu.py:830-891, :1128-1262, including its sign of a63)
 // ---------------------------------------------------------------------------
 __device__ __forceinline__ double eclip(double x) { return x < -700.0 ? -700.0 : (x > 700.0 ? 700.0 : x); }
 
@@ -193,8 +205,6 @@ __device__ __forceinline__ void ens_rhs(const EElem& e, double V0, double eta, c
     k[3] = dstdt;
 }
 
-// Dormand-Prince tableau as used by the reference (QDsim_gpu.py:1129-1159,
-// including its sign of a63)
 __device__ __forceinline__ void ens_dp45(const EElem& e, double V0, double eta, const double* y, double h,
                                          double* y5, double* y4) {
     double k1[4], k2[4], k3[4], k4[4], k5[4], k6[4], k7[4], t[4];
@@ -234,36 +244,8 @@ __device__ __forceinline__ double epy_min(double a, double b) { return b < a ? b
 __device__ __forceinline__ unsigned long long ebits(double x) { return (unsigned long long)__double_as_longlong(x); }
 __device__ __forceinline__ double eflt(unsigned long long u) { return __longlong_as_double((long long)u); }
 
-__device__ bool ens_barrier(EnsCtrl* c, unsigned nblocks, long long spin_limit) {
-    __shared__ int ok;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        ok = 1;
-        const unsigned g = __hip_atomic_load(&c->bar_gen, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
-        __threadfence();
-        const unsigned arrived = atomicAdd(&c->bar_count, 1u);
-        if (arrived == nblocks - 1) {
-            __hip_atomic_store(&c->bar_count, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            __hip_atomic_fetch_add(&c->bar_gen, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
-        } else {
-            long long spins = 0;
-            while (__hip_atomic_load(&c->bar_gen, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) == g) {
-                __builtin_amdgcn_s_sleep(2);
-                if (++spins > spin_limit) {
-                    atomicExch(&c->status, 2);
-                    ok = 0;
-                    break;
-                }
-            }
-        }
-        __threadfence();
-    }
-    __syncthreads();
-    return ok != 0;
-}
-
 // Max of (x, y) over the lanes whose member equals the wave's first member;
-// lanes of other members (a wave straddling two members) post their own atomics.
+// lanes of another member (a wave straddling two members) post their own atomics.
 __device__ __forceinline__ void member_max_atomic(int m, bool valid, unsigned long long x, unsigned long long y,
                                                   unsigned long long* slot_base /* [B][2] */) {
     const int m0 = __shfl(m, 0, 64);
@@ -273,11 +255,11 @@ __device__ __forceinline__ void member_max_atomic(int m, bool valid, unsigned lo
         unsigned long long vx = valid ? x : 0ull, vy = valid ? y : 0ull;
 #pragma unroll
         for (int o = 32; o > 0; o >>= 1) {
-            unsigned long long wx = __shfl_xor(vx, o, 64), wy = __shfl_xor(vy, o, 64);
+            const unsigned long long wx = __shfl_xor(vx, o, 64), wy = __shfl_xor(vy, o, 64);
             vx = wx > vx ? wx : vx;
             vy = wy > vy ? wy : vy;
         }
-        if ((threadIdx.x & 63) == 0) {
+        if ((threadIdx.x & 63) == 0 && (vx | vy)) {
             atomicMax(slot_base + 2 * m0, vx);
             atomicMax(slot_base + 2 * m0 + 1, vy);
         }
@@ -287,147 +269,140 @@ __device__ __forceinline__ void member_max_atomic(int m, bool valid, unsigned lo
     }
 }
 
-__global__ __launch_bounds__(256) void ens_rk_kernel(EnsArgs A) {
+__device__ __forceinline__ bool pending(const MemberCtrl& c) { return c.active && !c.accepted; }
+
+// One attempt for every item of a pending member; round 0 first sums the
+// GEMM's split-K partials into the frozen traction rates.
+__global__ __launch_bounds__(256) void ens_attempt_kernel(EnsArgs A, int round) {
     const long long n = A.n;
     const int B = A.B;
     const long long items = n * B;
-    const long long stride = (long long)gridDim.x * blockDim.x;
-    const long long t0 = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    const MemberCtrl* mc = A.mc + (size_t)(round & 1) * B;
+    // the previous round's reduction slot is free again (its readers have finished)
+    const int prev = (round + ENS_ROUNDS - 1) % ENS_ROUNDS;
+    if (t < 2 * B) A.slot[(size_t)prev * B * 2 + t] = 0ull;
+    const bool in = t < items;
+    const long long tt = in ? t : items - 1;
+    const int m = (int)(tt / n);
+    const bool pend = in && pending(mc[m]);
     double* f = A.f;
-    auto F = [&](int field, long long idx) -> double& { return f[(size_t)field * items + idx]; };
-    const int par = (int)(__hip_atomic_load(&A.ctrl->gsteps, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) & 1);
-    unsigned long long* slots = A.slot + (size_t)par * 24 * B * 2;
-    // Every workgroup keeps its own copy of the member control in LDS and
-    // applies the same deterministic accept/reject rule to the same reduced
-    // errors, so one grid barrier per attempt round suffices.
-    // s_st: 0 pending, 1 inactive, 2 accepted in this round, 3 failed, 4 accepted earlier
-    __shared__ double s_h[64], s_dtnext[64];
-    __shared__ int s_st[64], s_nrjct[64];
-    __shared__ int s_any;
-    if (threadIdx.x < B) {
-        const MemberCtrl c = A.mc[threadIdx.x];
-        s_h[threadIdx.x] = c.dttry;
-        s_dtnext[threadIdx.x] = c.dttry;
-        s_st[threadIdx.x] = c.active ? 0 : 1;
-        s_nrjct[threadIdx.x] = 0;
-    }
-    for (long long t = t0; t < 24 * B * 2; t += stride) A.slot[(size_t)(par ^ 1) * 24 * B * 2 + t] = 0ull;
-    // frozen traction rates (sum of the GEMM's split-K partials, fixed order);
-    // each item is produced and later consumed by the same thread
-    for (long long t = t0; t < items; t += stride) {
-        const int m = (int)(t / n);
-        const long long e = t - (long long)m * n;
-        double s0 = 0, s1 = 0, s2 = 0;
-        for (int c = 0; c < A.nchunk; ++c) {
-            s0 += A.partial[(((size_t)c * 3 + 0) * B + m) * A.npad + e];
-            s1 += A.partial[(((size_t)c * 3 + 1) * B + m) * A.npad + e];
-            s2 += A.partial[(((size_t)c * 3 + 2) * B + m) * A.npad + e];
-        }
-        F(E_ADOTV1, t) = s0;
-        F(E_ADOTV2, t) = s1;
-        F(E_DSIGMADT, t) = s2 + F(E_NORMAL_LOADING, t);
-    }
-    __syncthreads();
-
-    for (int att = 0; att < 22; ++att) {
-        for (long long base = (long long)blockIdx.x * blockDim.x; base < items; base += stride) {
-            const long long t = base + threadIdx.x;
-            const bool in = t < items;
-            const long long tt = in ? t : items - 1;
-            const int m = (int)(tt / n);
-            const bool pend = in && s_st[m] == 0;
-            unsigned long long r1b = 0ull, r2b = 0ull;
-            if (pend) {
-                EElem e;
-                e.a = F(E_A, tt); e.b = F(E_B, tt); e.dc = F(E_DC, tt); e.f0 = F(E_F0, tt);
-                e.P = F(E_P, tt); e.dPdt = F(E_DPDT0, tt); e.sl = F(E_SHEAR_LOADING, tt);
-                e.A1 = F(E_ADOTV1, tt); e.A2 = F(E_ADOTV2, tt); e.dsig = F(E_DSIGMADT, tt);
-                e.V = F(E_SLIPV, tt);
-                const double y[4] = {F(E_TNO, tt), F(E_TT1O, tt), F(E_TT2O, tt), F(E_STATE, tt)};
-                double y5[4], y4[4];
-                ens_dp45(e, A.V0, A.eta, y, s_h[m], y5, y4);
-                F(E_Y5_0, tt) = y5[0]; F(E_Y5_1, tt) = y5[1]; F(E_Y5_2, tt) = y5[2]; F(E_Y5_3, tt) = y5[3];
-                const double r1 = fabs(fabs(y4[3] - y5[3]) / y4[3]);
-                const double rt1 = fabs(fabs(y4[1] - y5[1]) / y4[1]);
-                const double rt2 = fabs(fabs(y4[2] - y5[2]) / y4[2]);
-                const double r2 = (rt2 > rt1 || rt2 != rt2) ? rt2 : rt1;
-                r1b = ebits(r1);
-                r2b = ebits(r2);
+    auto F = [&](int field) -> double& { return f[(size_t)field * items + tt]; };
+    unsigned long long r1b = 0ull, r2b = 0ull;
+    if (pend) {
+        if (round == 0) {
+            const long long e = tt - (long long)m * n;
+            double s0 = 0, s1 = 0, s2 = 0;
+            for (int c = 0; c < A.nchunk; ++c) {
+                s0 += A.partial[(((size_t)c * 3 + 0) * B + m) * A.npad + e];
+                s1 += A.partial[(((size_t)c * 3 + 1) * B + m) * A.npad + e];
+                s2 += A.partial[(((size_t)c * 3 + 2) * B + m) * A.npad + e];
             }
-            member_max_atomic(m, pend, r1b, r2b, slots + (size_t)att * B * 2);
+            F(E_ADOTV1) = s0;
+            F(E_ADOTV2) = s1;
+            F(E_DSIGMADT) = s2 + F(E_NORMAL_LOADING);
         }
-        if (!ens_barrier(A.ctrl, (unsigned)A.nblocks, A.spin_limit)) return;
-        // per-member decision (QDsim_gpu.py:959-985), replicated in every workgroup
-        if (threadIdx.x < B && s_st[threadIdx.x] == 0) {
-            const int m = threadIdx.x;
-            const double e1 = eflt(__hip_atomic_load(slots + ((size_t)att * B + m) * 2 + 0, __ATOMIC_RELAXED,
-                                                     __HIP_MEMORY_SCOPE_AGENT)) + 1e-10;
-            const double e2 = eflt(__hip_atomic_load(slots + ((size_t)att * B + m) * 2 + 1, __ATOMIC_RELAXED,
-                                                     __HIP_MEMORY_SCOPE_AGENT)) + 1e-10;
+        EElem e;
+        e.a = F(E_A); e.b = F(E_B); e.dc = F(E_DC); e.f0 = F(E_F0);
+        e.P = F(E_P); e.dPdt = F(E_DPDT0); e.sl = F(E_SHEAR_LOADING);
+        e.A1 = F(E_ADOTV1); e.A2 = F(E_ADOTV2); e.dsig = F(E_DSIGMADT);
+        e.V = F(E_SLIPV);
+        const double y[4] = {F(E_TNO), F(E_TT1O), F(E_TT2O), F(E_STATE)};
+        double y5[4], y4[4];
+        ens_dp45(e, A.V0, A.eta, y, mc[m].h, y5, y4);
+        F(E_Y5_0) = y5[0]; F(E_Y5_1) = y5[1]; F(E_Y5_2) = y5[2]; F(E_Y5_3) = y5[3];
+        const double r1 = fabs(fabs(y4[3] - y5[3]) / y4[3]);
+        const double rt1 = fabs(fabs(y4[1] - y5[1]) / y4[1]);
+        const double rt2 = fabs(fabs(y4[2] - y5[2]) / y4[2]);
+        const double r2 = (rt2 > rt1 || rt2 != rt2) ? rt2 : rt1;
+        r1b = ebits(r1);
+        r2b = ebits(r2);
+    }
+    member_max_atomic(m, pend, r1b, r2b, A.slot + (size_t)round * B * 2);
+}
+
+// Accept/reject per member (replicated in every workgroup) and post-step update
+// of the items of members accepted in this round (QDsim_gpu.py:959-1022).
+__global__ __launch_bounds__(256) void ens_decide_kernel(EnsArgs A, int round) {
+    const long long n = A.n;
+    const int B = A.B;
+    const long long items = n * B;
+    const MemberCtrl* mc_in = A.mc + (size_t)(round & 1) * B;
+    MemberCtrl* mc_out = A.mc + (size_t)((round + 1) & 1) * B;
+    __shared__ MemberCtrl s_mc[64];
+    __shared__ int s_acc[64];
+    if (threadIdx.x < B) {
+        const int m = threadIdx.x;
+        MemberCtrl c = mc_in[m];
+        int acc = 0;
+        if (pending(c)) {
+            const unsigned long long b1 = __hip_atomic_load(A.slot + ((size_t)round * B + m) * 2 + 0, __ATOMIC_RELAXED,
+                                                            __HIP_MEMORY_SCOPE_AGENT);
+            const unsigned long long b2 = __hip_atomic_load(A.slot + ((size_t)round * B + m) * 2 + 1, __ATOMIC_RELAXED,
+                                                            __HIP_MEMORY_SCOPE_AGENT);
+            const double e1 = eflt(b1) + 1e-10;
+            const double e2 = eflt(b2) + 1e-10;
             const double c1 = e1 / 1e-4, c2 = e2 / 1e-4;
-            const double h = s_h[m];
+            const double h = c.h;
             const double hn1 = h * 0.9 * pow(1e-4 / e1, 0.2);
             const double hn2 = h * 0.9 * pow(1e-4 / e2, 0.2);
             if (epy_max(c1, c2) < 1.0 && !(c1 != c1 || c2 != c2)) {
-                s_dtnext[m] = epy_min(1.5 * h, epy_min(hn1, hn2));
-                s_st[m] = 2;
+                acc = 1;
+                c.accepted = 1;
+                c.last_h = h;
+                c.dtnext = epy_min(1.5 * h, epy_min(hn1, hn2));
+                c.time = c.time + h;
+                c.steps += 1;
             } else {
-                s_nrjct[m] += 1;
-                const double hn = epy_max(0.5 * h, epy_min(hn1, hn2));
-                s_h[m] = hn;
-                if (hn < 1.e-15 || s_nrjct[m] > 20) s_st[m] = 3;
+                c.nrjct += 1;
+                c.h = epy_max(0.5 * h, epy_min(hn1, hn2));
+                if (c.h < 1.e-15 || c.nrjct > 20) {
+                    c.status = 1;
+                    c.active = 0;
+                }
             }
         }
-        __syncthreads();
-        // post-step update of members accepted in this round (QDsim_gpu.py:987-1022)
-        for (long long t = t0; t < items; t += stride) {
-            const int m = (int)(t / n);
-            if (s_st[m] != 2) continue;
-            const double h = s_h[m];
-            const double Tno = F(E_Y5_0, t), Tt1 = F(E_Y5_1, t), Tt2 = F(E_Y5_2, t), st = F(E_Y5_3, t);
-            const double a = F(E_A, t);
-            F(E_TNO, t) = Tno; F(E_TT1O, t) = Tt1; F(E_TT2O, t) = Tt2; F(E_STATE, t) = st;
+        s_mc[m] = c;
+        s_acc[m] = acc;
+    }
+    __syncthreads();
+    const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < items) {
+        const int m = (int)(t / n);
+        if (s_acc[m]) {
+            double* f = A.f;
+            auto F = [&](int field) -> double& { return f[(size_t)field * items + t]; };
+            const double h = s_mc[m].last_h;
+            const double Tno = F(E_Y5_0), Tt1 = F(E_Y5_1), Tt2 = F(E_Y5_2), st = F(E_Y5_3);
+            const double a = F(E_A);
+            F(E_TNO) = Tno; F(E_TT1O) = Tt1; F(E_TT2O) = Tt2; F(E_STATE) = st;
             const double Tt = sqrt(Tt1 * Tt1 + Tt2 * Tt2);
-            F(E_TT, t) = Tt;
+            F(E_TT) = Tt;
             const double ex = 2.0 * A.V0 * exp(-st / a);
             const double v1 = ex * sinh(Tt1 / Tno / a), v2 = ex * sinh(Tt2 / Tno / a);
-            F(E_SLIPV1, t) = v1; F(E_SLIPV2, t) = v2;
+            F(E_SLIPV1) = v1; F(E_SLIPV2) = v2;
             double v = sqrt(v1 * v1 + v2 * v2);
-            F(E_RAKE, t) = atan2(v2, v1);
+            F(E_RAKE) = atan2(v2, v1);
             if (v < 1e-30) v = 1e-30;
-            F(E_SLIPV, t) = v;
-            const double s1 = F(E_SLIP1, t) + v1 * h, s2 = F(E_SLIP2, t) + v2 * h;
-            F(E_SLIP1, t) = s1; F(E_SLIP2, t) = s2; F(E_SLIP, t) = sqrt(s1 * s1 + s2 * s2);
-            F(E_FRIC, t) = Tt / (Tno - F(E_P, t) * 1e-6);
+            F(E_SLIPV) = v;
+            const double s1 = F(E_SLIP1) + v1 * h, s2 = F(E_SLIP2) + v2 * h;
+            F(E_SLIP1) = s1; F(E_SLIP2) = s2; F(E_SLIP) = sqrt(s1 * s1 + s2 * s2);
+            F(E_FRIC) = Tt / (Tno - F(E_P) * 1e-6);
         }
-        __syncthreads();
-        if (threadIdx.x < B && s_st[threadIdx.x] == 2) s_st[threadIdx.x] = 4;
-        __syncthreads();
-        if (threadIdx.x == 0) {
-            int any = 0;
-            for (int m = 0; m < B; ++m) any |= (s_st[m] == 0);
-            s_any = any;
-        }
-        __syncthreads();
-        if (!s_any) break;
     }
     if (blockIdx.x == 0 && threadIdx.x < B) {
-        const int m = threadIdx.x;
-        MemberCtrl& c = A.mc[m];
-        c.nrjct = s_nrjct[m];
-        if (s_st[m] == 4) {
-            c.time = c.time + s_h[m];
-            c.h = s_h[m];
-            c.dtnext = s_dtnext[m];
-            c.dttry = s_dtnext[m];
-            c.steps += 1;
-        } else if (s_st[m] == 3 || s_st[m] == 0) {
-            c.h = s_h[m];
-            c.status = 1;
-            c.active = 0;
+        MemberCtrl c = s_mc[threadIdx.x];
+        if (s_acc[threadIdx.x]) {
+            c.h = c.dtnext;
+            c.nrjct = 0;
+            if (c.t_end > 0.0 && c.time >= c.t_end) {
+                c.finished = 1;
+                c.active = 0;
+                c.finish_gstep = A.ctrl->gsteps;
+            }
         }
+        mc_out[threadIdx.x] = c;
     }
-    if (t0 == 0) A.ctrl->gsteps += 1;
 }
 
 __global__ void ens_trig_kernel(const double* __restrict__ r0, double* __restrict__ c0, double* __restrict__ s0,
@@ -441,17 +416,17 @@ __global__ void ens_trig_kernel(const double* __restrict__ r0, double* __restric
 
 struct Ensemble {
     long long n = 0, ld = 0, npad = 0;
-    int B = 64, kc = 512, nchunk = 0, nblocks = 0;
+    int B = 64, kc = 0, nchunk = 0;
     const float* mats[6] = {nullptr};
     float* own_mats[6] = {nullptr};
     double* f = nullptr;
     float* U = nullptr;        // 2 x n x B
     float* partial = nullptr;  // nchunk x 3 x B x npad
-    MemberCtrl* mc = nullptr;
+    MemberCtrl* mc = nullptr;  // 2 x B
     unsigned long long* slot = nullptr;
     EnsCtrl* ctrl = nullptr;
     double V0 = 1e-6, Vpl = 1e-9, eta = 0.0;
-    long long spin_limit = 20000000;
+    hipStream_t stream = nullptr;
     hipGraphExec_t graph = nullptr;
     int graph_steps = 0;
 };
@@ -467,11 +442,9 @@ static EnsArgs ens_args(Ensemble* e) {
     a.npad = e->npad;
     a.B = e->B;
     a.nchunk = e->nchunk;
-    a.nblocks = e->nblocks;
     a.V0 = e->V0;
     a.Vpl = e->Vpl;
     a.eta = e->eta;
-    a.spin_limit = e->spin_limit;
     return a;
 }
 
@@ -482,7 +455,7 @@ static void ens_launch_gemm(Ensemble* e, hipStream_t st) {
     float* U2 = e->U + (size_t)e->n * e->B;
     hipLaunchKernelGGL(ens_prep_u_kernel, dim3((unsigned)((items + 255) / 256)), dim3(256), 0, st,
                        f + (size_t)E_SLIPV1 * items, f + (size_t)E_SLIPV2 * items, f + (size_t)E_COSR0 * items,
-                       f + (size_t)E_SINR0 * items, e->Vpl, e->n, e->B, e->mc, U1, U2);
+                       f + (size_t)E_SINR0 * items, e->Vpl, e->n, e->B, e->mc, e->ctrl, U1, U2);
     dim3 grid((unsigned)((e->n + 63) / 64), (unsigned)e->nchunk);
     hipLaunchKernelGGL(ens_gemm_kernel, grid, dim3(64), 0, st, e->mats[0], e->mats[1], e->mats[2], e->mats[3],
                        e->mats[4], e->mats[5], e->ld, e->n, e->kc, U1, U2, e->partial, e->npad);
@@ -490,7 +463,19 @@ static void ens_launch_gemm(Ensemble* e, hipStream_t st) {
 
 static void ens_launch_step(Ensemble* e, hipStream_t st) {
     ens_launch_gemm(e, st);
-    hipLaunchKernelGGL(ens_rk_kernel, dim3((unsigned)e->nblocks), dim3(256), 0, st, ens_args(e));
+    const EnsArgs a = ens_args(e);
+    const unsigned blocks = (unsigned)((e->n * e->B + 255) / 256);
+    for (int r = 0; r < ENS_ROUNDS; ++r) {
+        hipLaunchKernelGGL(ens_attempt_kernel, dim3(blocks), dim3(256), 0, st, a, r);
+        hipLaunchKernelGGL(ens_decide_kernel, dim3(blocks), dim3(256), 0, st, a, r);
+    }
+}
+
+static void ens_drop_graph(Ensemble* e) {
+    if (e->graph) {
+        (void)hipGraphExecDestroy(e->graph);
+        e->graph = nullptr;
+    }
 }
 
 }  // namespace pq3d
@@ -502,6 +487,7 @@ extern "C" {
 // B members (B == 64) on an n-element mesh.  If d_mats is non-null the six
 // FP32 source-major matrices (leading dimension ld, multiple of 4) are
 // borrowed from the caller (e.g. a dense simulation), else allocated here.
+// kc <= 0 chooses the split-K chunk size from the device's concurrent slots.
 int pq3d_ens_create(void** out, long long n, int B, void* const* d_mats, long long ld, int kc) {
     clear_error();
     PQ_CHECK(B == 64, "pq3d_ens_create: B must be 64 (got %d)", B);
@@ -512,9 +498,9 @@ int pq3d_ens_create(void** out, long long n, int B, void* const* d_mats, long lo
     if (kc > 0) {
         e->kc = (kc + 3) / 4 * 4;
     } else {
-        // choose the number of column chunks so that the one-wave GEMM tiles
-        // (64 rows x chunk) fill whole waves of the device's concurrent slots,
-        // with a small penalty on the split-K partial traffic
+        // chunks such that the one-wave GEMM tiles (64 rows x chunk) fill whole
+        // waves of the device's concurrent slots, with a small penalty on the
+        // split-K partial traffic
         int dev = 0, per_cu = 0;
         PQ_HIP(hipGetDevice(&dev));
         hipDeviceProp_t p;
@@ -536,7 +522,7 @@ int pq3d_ens_create(void** out, long long n, int B, void* const* d_mats, long lo
         e->kc = (int)(((n + best_nc - 1) / best_nc + 3) / 4 * 4);
     }
     e->nchunk = (int)((n + e->kc - 1) / e->kc);
-    e->npad = (n + 255) / 256 * 256;
+    e->npad = (n + 63) / 64 * 64;
     if (d_mats) {
         PQ_CHECK(ld >= n && ld % 4 == 0, "pq3d_ens_create: ld must be >= n and a multiple of 4");
         e->ld = ld;
@@ -550,36 +536,25 @@ int pq3d_ens_create(void** out, long long n, int B, void* const* d_mats, long lo
         }
     }
     const long long items = n * B;
+    PQ_HIP(hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking));
     PQ_HIP(hipMalloc((void**)&e->f, (size_t)E_NFIELDS * items * sizeof(double)));
     PQ_HIP(hipMemset(e->f, 0, (size_t)E_NFIELDS * items * sizeof(double)));
     PQ_HIP(hipMalloc((void**)&e->U, (size_t)2 * items * sizeof(float)));
     PQ_HIP(hipMalloc((void**)&e->partial, (size_t)e->nchunk * 3 * B * e->npad * sizeof(float)));
     PQ_HIP(hipMemset(e->partial, 0, (size_t)e->nchunk * 3 * B * e->npad * sizeof(float)));
-    PQ_HIP(hipMalloc((void**)&e->mc, B * sizeof(MemberCtrl)));
-    PQ_HIP(hipMemset(e->mc, 0, B * sizeof(MemberCtrl)));
-    PQ_HIP(hipMalloc((void**)&e->slot, (size_t)2 * 24 * B * 2 * sizeof(unsigned long long)));
-    PQ_HIP(hipMemset(e->slot, 0, (size_t)2 * 24 * B * 2 * sizeof(unsigned long long)));
+    PQ_HIP(hipMalloc((void**)&e->mc, 2 * B * sizeof(MemberCtrl)));
+    PQ_HIP(hipMalloc((void**)&e->slot, (size_t)ENS_ROUNDS * B * 2 * sizeof(unsigned long long)));
+    PQ_HIP(hipMemset(e->slot, 0, (size_t)ENS_ROUNDS * B * 2 * sizeof(unsigned long long)));
     PQ_HIP(hipMalloc((void**)&e->ctrl, sizeof(EnsCtrl)));
     PQ_HIP(hipMemset(e->ctrl, 0, sizeof(EnsCtrl)));
-    // persistent RK grid: every workgroup must be co-resident
-    int dev = 0;
-    PQ_HIP(hipGetDevice(&dev));
-    hipDeviceProp_t p;
-    PQ_HIP(hipGetDeviceProperties(&p, dev));
-    int per_cu = 0;
-    PQ_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, ens_rk_kernel, 256, 0));
-    long long want = (items + 255) / 256;
-    long long cap = (long long)per_cu * p.multiProcessorCount;
-    e->nblocks = (int)(want < cap ? want : cap);
-    PQ_CHECK(e->nblocks > 0, "pq3d_ens_create: no co-resident workgroups");
-    // members start active
-    MemberCtrl h[64];
+    // members start active with the reference's first trial step (QDsim_gpu.py:149)
+    MemberCtrl h[2 * 64];
     memset(h, 0, sizeof(h));
-    for (int m = 0; m < B; ++m) {
+    for (int m = 0; m < 2 * B; ++m) {
         h[m].active = 1;
-        h[m].dttry = 1e-3;
+        h[m].h = 1e-3;
     }
-    PQ_HIP(hipMemcpy(e->mc, h, B * sizeof(MemberCtrl), hipMemcpyHostToDevice));
+    PQ_HIP(hipMemcpy(e->mc, h, 2 * B * sizeof(MemberCtrl), hipMemcpyHostToDevice));
     *out = e;
     return 0;
 }
@@ -588,7 +563,8 @@ int pq3d_ens_destroy(void* h) {
     clear_error();
     Ensemble* e = (Ensemble*)h;
     if (!e) return 0;
-    if (e->graph) (void)hipGraphExecDestroy(e->graph);
+    if (e->stream) (void)hipStreamSynchronize(e->stream);
+    ens_drop_graph(e);
     for (int k = 0; k < 6; ++k) (void)hipFree(e->own_mats[k]);
     (void)hipFree(e->f);
     (void)hipFree(e->U);
@@ -596,6 +572,7 @@ int pq3d_ens_destroy(void* h) {
     (void)hipFree(e->mc);
     (void)hipFree(e->slot);
     (void)hipFree(e->ctrl);
+    if (e->stream) (void)hipStreamDestroy(e->stream);
     delete e;
     return 0;
 }
@@ -608,6 +585,15 @@ int pq3d_ens_matrix_ptrs(void* h, void** ptrs, long long* ld) {
     return 0;
 }
 
+static int ens_trig(Ensemble* e, long long off, long long cnt) {
+    const long long items = e->n * e->B;
+    hipLaunchKernelGGL(ens_trig_kernel, dim3((unsigned)((cnt + 255) / 256)), dim3(256), 0, e->stream,
+                       e->f + (size_t)E_RAKE0 * items + off, e->f + (size_t)E_COSR0 * items + off,
+                       e->f + (size_t)E_SINR0 * items + off, cnt);
+    PQ_HIP(hipGetLastError());
+    return 0;
+}
+
 // Field values for all members: host array B x n (member-major).  Field
 // indices are those of the dense simulation.
 int pq3d_ens_set_field(void* h, int field, const double* host) {
@@ -615,14 +601,10 @@ int pq3d_ens_set_field(void* h, int field, const double* host) {
     Ensemble* e = (Ensemble*)h;
     PQ_CHECK(field >= 0 && field < E_Y5_0, "bad field %d", field);
     const long long items = e->n * e->B;
-    double* dst = e->f + (size_t)field * items;
-    PQ_HIP(hipMemcpy(dst, host, items * sizeof(double), hipMemcpyHostToDevice));
-    if (field == E_RAKE0) {
-        hipLaunchKernelGGL(ens_trig_kernel, dim3((unsigned)((items + 255) / 256)), dim3(256), 0, 0, dst,
-                           e->f + (size_t)E_COSR0 * items, e->f + (size_t)E_SINR0 * items, items);
-        PQ_HIP(hipGetLastError());
-        PQ_HIP(hipDeviceSynchronize());
-    }
+    PQ_HIP(hipMemcpyAsync(e->f + (size_t)field * items, host, items * sizeof(double), hipMemcpyHostToDevice,
+                          e->stream));
+    if (field == E_RAKE0 && ens_trig(e, 0, items)) return -1;
+    PQ_HIP(hipStreamSynchronize(e->stream));
     return 0;
 }
 
@@ -631,7 +613,35 @@ int pq3d_ens_get_field(void* h, int field, double* host) {
     Ensemble* e = (Ensemble*)h;
     PQ_CHECK(field >= 0 && field < E_NFIELDS, "bad field %d", field);
     const long long items = e->n * e->B;
-    PQ_HIP(hipMemcpy(host, e->f + (size_t)field * items, items * sizeof(double), hipMemcpyDeviceToHost));
+    PQ_HIP(hipMemcpyAsync(host, e->f + (size_t)field * items, items * sizeof(double), hipMemcpyDeviceToHost,
+                          e->stream));
+    PQ_HIP(hipStreamSynchronize(e->stream));
+    return 0;
+}
+
+// One member's field (host array of n): slot `m`.
+int pq3d_ens_set_member_field(void* h, int m, int field, const double* host_n) {
+    clear_error();
+    Ensemble* e = (Ensemble*)h;
+    PQ_CHECK(m >= 0 && m < e->B, "bad member %d", m);
+    PQ_CHECK(field >= 0 && field < E_Y5_0, "bad field %d", field);
+    const long long items = e->n * e->B;
+    PQ_HIP(hipMemcpyAsync(e->f + (size_t)field * items + (size_t)m * e->n, host_n, e->n * sizeof(double),
+                          hipMemcpyHostToDevice, e->stream));
+    if (field == E_RAKE0 && ens_trig(e, (long long)m * e->n, e->n)) return -1;
+    PQ_HIP(hipStreamSynchronize(e->stream));
+    return 0;
+}
+
+int pq3d_ens_get_member_field(void* h, int m, int field, double* host_n) {
+    clear_error();
+    Ensemble* e = (Ensemble*)h;
+    PQ_CHECK(m >= 0 && m < e->B, "bad member %d", m);
+    PQ_CHECK(field >= 0 && field < E_NFIELDS, "bad field %d", field);
+    const long long items = e->n * e->B;
+    PQ_HIP(hipMemcpyAsync(host_n, e->f + (size_t)field * items + (size_t)m * e->n, e->n * sizeof(double),
+                          hipMemcpyDeviceToHost, e->stream));
+    PQ_HIP(hipStreamSynchronize(e->stream));
     return 0;
 }
 
@@ -641,60 +651,66 @@ int pq3d_ens_set_params(void* h, double V0, double Vpl, double eta) {
     e->V0 = V0;
     e->Vpl = Vpl;
     e->eta = eta;
-    if (e->graph) {
-        (void)hipGraphExecDestroy(e->graph);
-        e->graph = nullptr;
-    }
+    ens_drop_graph(e);
     return 0;
 }
 
-// Member control: per member 5 doubles (time, next dt, last h, last dtnext,
-// unused) and 4 ints (rejections in the last step, status, active, steps).
-int pq3d_ens_members(void* h, double* d5, long long* i4) {
+// Member control (the half read at the start of a global step).  Per member:
+// d6 = time, next trial step, last accepted step, last dtnext, t_end, 0;
+// i6 = rejections since the last accepted step, status, active, finished,
+// accepted steps, global step of finishing.
+int pq3d_ens_members(void* h, double* d6, long long* i6) {
     clear_error();
     Ensemble* e = (Ensemble*)h;
     MemberCtrl c[64];
-    PQ_HIP(hipMemcpy(c, e->mc, e->B * sizeof(MemberCtrl), hipMemcpyDeviceToHost));
+    PQ_HIP(hipMemcpyAsync(c, e->mc, e->B * sizeof(MemberCtrl), hipMemcpyDeviceToHost, e->stream));
+    PQ_HIP(hipStreamSynchronize(e->stream));
     for (int m = 0; m < e->B; ++m) {
-        d5[5 * m + 0] = c[m].time;
-        d5[5 * m + 1] = c[m].dttry;
-        d5[5 * m + 2] = c[m].h;
-        d5[5 * m + 3] = c[m].dtnext;
-        d5[5 * m + 4] = 0.0;
-        i4[4 * m + 0] = c[m].nrjct;
-        i4[4 * m + 1] = c[m].status;
-        i4[4 * m + 2] = c[m].active;
-        i4[4 * m + 3] = c[m].steps;
+        d6[6 * m + 0] = c[m].time;
+        d6[6 * m + 1] = c[m].h;
+        d6[6 * m + 2] = c[m].last_h;
+        d6[6 * m + 3] = c[m].dtnext;
+        d6[6 * m + 4] = c[m].t_end;
+        d6[6 * m + 5] = 0.0;
+        i6[6 * m + 0] = c[m].nrjct;
+        i6[6 * m + 1] = c[m].status;
+        i6[6 * m + 2] = c[m].active;
+        i6[6 * m + 3] = c[m].finished;
+        i6[6 * m + 4] = c[m].steps;
+        i6[6 * m + 5] = c[m].finish_gstep;
     }
     return 0;
 }
 
-// Set each member's next step size and active flag (host arrays of B).
-int pq3d_ens_set_members(void* h, const double* dttry, const int* active) {
+// (Re)start one slot: time 0, trial step h0, finish at t_end (0: never),
+// active flag.  Fields must be set separately.
+int pq3d_ens_reset_member(void* h, int m, double h0, double t_end, int active) {
     clear_error();
     Ensemble* e = (Ensemble*)h;
-    MemberCtrl c[64];
-    PQ_HIP(hipMemcpy(c, e->mc, e->B * sizeof(MemberCtrl), hipMemcpyDeviceToHost));
-    for (int m = 0; m < e->B; ++m) {
-        if (dttry) c[m].dttry = dttry[m];
-        if (active) c[m].active = active[m];
-    }
-    PQ_HIP(hipMemcpy(e->mc, c, e->B * sizeof(MemberCtrl), hipMemcpyHostToDevice));
+    PQ_CHECK(m >= 0 && m < e->B, "bad member %d", m);
+    MemberCtrl c;
+    memset(&c, 0, sizeof(c));
+    c.h = h0;
+    c.t_end = t_end;
+    c.active = active;
+    PQ_HIP(hipMemcpyAsync(e->mc + m, &c, sizeof(c), hipMemcpyHostToDevice, e->stream));
+    PQ_HIP(hipMemcpyAsync(e->mc + e->B + m, &c, sizeof(c), hipMemcpyHostToDevice, e->stream));
+    PQ_HIP(hipStreamSynchronize(e->stream));
     return 0;
 }
 
-// Only the GEMM (u from the current slip rates -> per-member traction rates
-// into AdotV1/AdotV2/dsigmadt without normal loading); for tests and timing.
+// Only the GEMM (traction rates of all members from their current slip
+// rates); `reps` timed repetitions on the ensemble's stream.
 int pq3d_ens_gemm(void* h, int reps, float* ms_per) {
     clear_error();
     Ensemble* e = (Ensemble*)h;
     hipEvent_t a, b;
     PQ_HIP(hipEventCreate(&a));
     PQ_HIP(hipEventCreate(&b));
-    ens_launch_gemm(e, 0);
-    PQ_HIP(hipEventRecord(a, 0));
-    for (int r = 0; r < reps; ++r) ens_launch_gemm(e, 0);
-    PQ_HIP(hipEventRecord(b, 0));
+    ens_launch_gemm(e, e->stream);
+    PQ_HIP(hipEventRecord(a, e->stream));
+    for (int r = 0; r < reps; ++r) ens_launch_gemm(e, e->stream);
+    PQ_HIP(hipEventRecord(b, e->stream));
     PQ_HIP(hipEventSynchronize(b));
     float ms = 0;
     PQ_HIP(hipEventElapsedTime(&ms, a, b));
@@ -704,8 +720,6 @@ int pq3d_ens_gemm(void* h, int reps, float* ms_per) {
     return 0;
 }
 
-// Copy the summed GEMM result (3 x B x n, FP64 host) -- sums the partials on
-// the host side of the call through a small device pass.
 __global__ void ens_sum_partials_kernel(const float* __restrict__ partial, int nchunk, int B, long long n,
                                         long long npad, double* __restrict__ out) {
     const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -718,30 +732,32 @@ __global__ void ens_sum_partials_kernel(const float* __restrict__ partial, int n
     out[t] = s;
 }
 
+// The last GEMM's result summed over chunks: host array 3 x B x n.
 int pq3d_ens_gemm_result(void* h, double* host_3Bn) {
     clear_error();
     Ensemble* e = (Ensemble*)h;
     const long long tot = 3LL * e->B * e->n;
     double* d = nullptr;
     PQ_HIP(hipMalloc((void**)&d, tot * sizeof(double)));
-    hipLaunchKernelGGL(ens_sum_partials_kernel, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, 0, e->partial,
-                       e->nchunk, e->B, e->n, e->npad, d);
-    hipError_t err = hipMemcpy(host_3Bn, d, tot * sizeof(double), hipMemcpyDeviceToHost);
+    hipLaunchKernelGGL(ens_sum_partials_kernel, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, e->stream,
+                       e->partial, e->nchunk, e->B, e->n, e->npad, d);
+    hipError_t err = hipMemcpyAsync(host_3Bn, d, tot * sizeof(double), hipMemcpyDeviceToHost, e->stream);
+    if (err == hipSuccess) err = hipStreamSynchronize(e->stream);
     (void)hipFree(d);
     PQ_HIP(err);
     return 0;
 }
 
-// nsteps global steps (each member takes one accepted step per global step),
-// no host synchronisation in between; HIP graph of graph_steps steps.
-int pq3d_ens_run(void* h, long long nsteps, int graph_steps, int* status) {
+// Enqueue nsteps global steps on the ensemble's stream (HIP graph of
+// graph_steps steps); waits for completion unless async != 0.
+int pq3d_ens_run(void* h, long long nsteps, int graph_steps, int async) {
     clear_error();
     Ensemble* e = (Ensemble*)h;
     if (graph_steps <= 0) graph_steps = 1;
     long long done = 0;
     if (nsteps >= graph_steps) {
         if (!e->graph || e->graph_steps != graph_steps) {
-            if (e->graph) (void)hipGraphExecDestroy(e->graph);
+            ens_drop_graph(e);
             hipStream_t cs;
             PQ_HIP(hipStreamCreateWithFlags(&cs, hipStreamNonBlocking));
             PQ_HIP(hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal));
@@ -753,14 +769,35 @@ int pq3d_ens_run(void* h, long long nsteps, int graph_steps, int* status) {
             (void)hipStreamDestroy(cs);
             e->graph_steps = graph_steps;
         }
-        for (; done + graph_steps <= nsteps; done += graph_steps) PQ_HIP(hipGraphLaunch(e->graph, 0));
+        for (; done + graph_steps <= nsteps; done += graph_steps) PQ_HIP(hipGraphLaunch(e->graph, e->stream));
     }
-    for (; done < nsteps; ++done) ens_launch_step(e, 0);
+    for (; done < nsteps; ++done) ens_launch_step(e, e->stream);
     PQ_HIP(hipGetLastError());
-    PQ_HIP(hipDeviceSynchronize());
-    EnsCtrl c;
-    PQ_HIP(hipMemcpy(&c, e->ctrl, sizeof(EnsCtrl), hipMemcpyDeviceToHost));
-    *status = c.status;
+    if (!async) PQ_HIP(hipStreamSynchronize(e->stream));
+    return 0;
+}
+
+int pq3d_ens_sync(void* h) {
+    clear_error();
+    Ensemble* e = (Ensemble*)h;
+    PQ_HIP(hipStreamSynchronize(e->stream));
+    return 0;
+}
+
+// 1 if the ensemble's queued work has completed, 0 if still running.
+int pq3d_ens_query(void* h, int* done) {
+    clear_error();
+    Ensemble* e = (Ensemble*)h;
+    hipError_t q = hipStreamQuery(e->stream);
+    if (q == hipSuccess) {
+        *done = 1;
+        return 0;
+    }
+    if (q == hipErrorNotReady) {
+        *done = 0;
+        return 0;
+    }
+    PQ_HIP(q);
     return 0;
 }
 

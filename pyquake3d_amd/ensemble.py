@@ -1,11 +1,15 @@
 """Ensemble of simulations on one shared mesh (batched GEMM on MFMA).
 
-``Ensemble(n, B=64, matrices_from=dense_sim)`` holds B members, each with its
-own friction parameters and state, sharing one set of FP32 traction Green's
-functions (borrowed from a ``DenseSim`` with FP32 storage, or owned).  One
-global step = one accepted Dormand-Prince step for every active member, each
-with its own step size (the reference's accept/reject rule per member,
-QDsim_gpu.py:959-985); members that fail are masked out.
+``Ensemble(n, B=64, matrices_from=dense_sim)`` holds B slots ("members"), each
+with its own friction parameters and state, sharing one set of FP32 traction
+Green's functions (borrowed from a ``DenseSim`` with FP32 storage, or owned).
+One global step = one GEMM for all members, then each active member's
+Dormand-Prince attempts with its own step size under the reference's
+accept/reject rule (QDsim_gpu.py:959-985); a member that has not accepted
+after the step's attempt rounds continues at the next global step, and a
+member that fails is masked out.  All work of one ensemble runs on its own
+HIP stream (``run(..., wait=False)`` returns immediately), so ensembles of
+different meshes overlap on the device.
 
 Field arrays are (B, n) float64 with the names of ``QDsim_gpu.FIELDS``.
 """
@@ -26,12 +30,16 @@ def _declare(L):
     _lib._sig(L, "pq3d_ens_matrix_ptrs", [vp, ctypes.POINTER(vp), ctypes.POINTER(ll)])
     _lib._sig(L, "pq3d_ens_set_field", [vp, i, dp])
     _lib._sig(L, "pq3d_ens_get_field", [vp, i, dp])
+    _lib._sig(L, "pq3d_ens_set_member_field", [vp, i, i, dp])
+    _lib._sig(L, "pq3d_ens_get_member_field", [vp, i, i, dp])
     _lib._sig(L, "pq3d_ens_set_params", [vp, d, d, d])
     _lib._sig(L, "pq3d_ens_members", [vp, dp, ctypes.POINTER(ll)])
-    _lib._sig(L, "pq3d_ens_set_members", [vp, dp, ctypes.POINTER(ctypes.c_int)])
+    _lib._sig(L, "pq3d_ens_reset_member", [vp, i, d, d, i])
     _lib._sig(L, "pq3d_ens_gemm", [vp, i, ctypes.POINTER(ctypes.c_float)])
     _lib._sig(L, "pq3d_ens_gemm_result", [vp, dp])
-    _lib._sig(L, "pq3d_ens_run", [vp, ll, i, ctypes.POINTER(ctypes.c_int)])
+    _lib._sig(L, "pq3d_ens_run", [vp, ll, i, i])
+    _lib._sig(L, "pq3d_ens_sync", [vp])
+    _lib._sig(L, "pq3d_ens_query", [vp, ctypes.POINTER(ctypes.c_int)])
 
 
 _lib.register(_declare)
@@ -58,6 +66,7 @@ class Ensemble:
         self.mat_ptrs = [int(p) for p in ptrs]
         self.ld = ld.value
 
+    # ---- fields
     def set(self, name, arr):
         a = np.ascontiguousarray(np.broadcast_to(np.asarray(arr, dtype=np.float64), (self.B, self.n)))
         check(_lib.lib().pq3d_ens_set_field(self.h, FIELD_ID[name], _lib.ptr(a)), "ens set " + name)
@@ -67,24 +76,35 @@ class Ensemble:
         check(_lib.lib().pq3d_ens_get_field(self.h, FIELD_ID[name], _lib.ptr(out)), "ens get " + name)
         return out
 
+    def set_member(self, m, name, arr):
+        a = np.ascontiguousarray(np.broadcast_to(np.asarray(arr, dtype=np.float64), (self.n,)))
+        check(_lib.lib().pq3d_ens_set_member_field(self.h, int(m), FIELD_ID[name], _lib.ptr(a)),
+              "ens set member " + name)
+
+    def get_member(self, m, name):
+        out = np.empty(self.n)
+        check(_lib.lib().pq3d_ens_get_member_field(self.h, int(m), FIELD_ID[name], _lib.ptr(out)),
+              "ens get member " + name)
+        return out
+
     def set_params(self, V0, Vpl, eta):
         check(_lib.lib().pq3d_ens_set_params(self.h, V0, Vpl, eta), "ens params")
 
+    # ---- member control
     def members(self):
-        d = np.zeros((self.B, 5))
-        i = np.zeros((self.B, 4), dtype=np.int64)
+        d = np.zeros((self.B, 6))
+        i = np.zeros((self.B, 6), dtype=np.int64)
         check(_lib.lib().pq3d_ens_members(self.h, _lib.ptr(d), i.ctypes.data_as(ctypes.POINTER(ctypes.c_longlong))),
               "ens members")
-        return {"time": d[:, 0], "dttry": d[:, 1], "h": d[:, 2], "dtnext": d[:, 3], "nrjct": i[:, 0],
-                "status": i[:, 1], "active": i[:, 2], "steps": i[:, 3]}
+        return {"time": d[:, 0], "dttry": d[:, 1], "h": d[:, 2], "dtnext": d[:, 3], "t_end": d[:, 4],
+                "nrjct": i[:, 0], "status": i[:, 1], "active": i[:, 2], "finished": i[:, 3], "steps": i[:, 4],
+                "finish_gstep": i[:, 5]}
 
-    def set_members(self, dttry=None, active=None):
-        dt = None if dttry is None else np.ascontiguousarray(np.broadcast_to(dttry, (self.B,)), dtype=np.float64)
-        ac = None if active is None else np.ascontiguousarray(np.broadcast_to(active, (self.B,)), dtype=np.int32)
-        check(_lib.lib().pq3d_ens_set_members(self.h, _lib.ptr(dt) if dt is not None else None,
-                                              ac.ctypes.data_as(ctypes.POINTER(ctypes.c_int)) if ac is not None
-                                              else None), "ens set members")
+    def reset_member(self, m, h0=1e-3, t_end=0.0, active=True):
+        check(_lib.lib().pq3d_ens_reset_member(self.h, int(m), float(h0), float(t_end), int(bool(active))),
+              "ens reset member")
 
+    # ---- execution
     def gemm(self, reps=0):
         f = ctypes.c_float()
         check(_lib.lib().pq3d_ens_gemm(self.h, int(reps), ctypes.byref(f)), "ens gemm")
@@ -95,10 +115,16 @@ class Ensemble:
         check(_lib.lib().pq3d_ens_gemm_result(self.h, _lib.ptr(out)), "ens gemm result")
         return out
 
-    def run(self, nsteps, graph_steps=4):
-        st = ctypes.c_int()
-        check(_lib.lib().pq3d_ens_run(self.h, int(nsteps), int(graph_steps), ctypes.byref(st)), "ens run")
-        return st.value
+    def run(self, nsteps, graph_steps=4, wait=True):
+        check(_lib.lib().pq3d_ens_run(self.h, int(nsteps), int(graph_steps), 0 if wait else 1), "ens run")
+
+    def sync(self):
+        check(_lib.lib().pq3d_ens_sync(self.h), "ens sync")
+
+    def done(self):
+        d = ctypes.c_int()
+        check(_lib.lib().pq3d_ens_query(self.h, ctypes.byref(d)), "ens query")
+        return bool(d.value)
 
     def close(self):
         if self.h:
@@ -112,20 +138,34 @@ class Ensemble:
             pass
 
 
-def init_members(ens, cols_per_member, V0, Vpl, mu, Cs):
-    """Steady-state initial condition per member from init-file columns
-    (B, n, 10) in the reference's layout (QDsim_gpu.py:714-770)."""
-    cols = np.asarray(cols_per_member, dtype=np.float64)
-    rake, a, b, dc, f0, _, Tn, V, sl, nl = np.moveaxis(cols, 2, 0)
+def steady_state_fields(cols, V0, Vpl):
+    """Field values of one member from its init-file columns (n, 10) in the
+    reference's layout (QDsim_gpu.py:714-770)."""
+    rake, a, b, dc, f0, _, Tn, V, sl, nl = np.asarray(cols, dtype=np.float64).T
     Tt = Tn * a * np.arcsinh(V / (2.0 * V0) * np.exp((f0 + b * np.log(V0 / Vpl)) / a))
     state = np.log(np.sinh(Tt / Tn / a) * 2.0 * V0 / V) * a
     x, y = np.cos(rake), np.sin(rake)
     v1 = V * x + 1e-16
     v2 = V * y + 1e-16
-    vals = {"Tno": Tn, "Tt1o": Tt * x, "Tt2o": Tt * y, "state": state, "slipv1": v1, "slipv2": v2,
-            "slipv": np.sqrt(v1 * v1 + v2 * v2), "a": a, "b": b, "dc": dc, "f0": f0, "shear_loading": sl,
-            "normal_loading": nl}
-    for k, v in vals.items():
-        ens.set(k, v)
-    ens.set("rake0", rake)
+    z = np.zeros_like(a)
+    return {"Tno": Tn, "Tt1o": Tt * x, "Tt2o": Tt * y, "state": state, "slipv1": v1, "slipv2": v2,
+            "slipv": np.sqrt(v1 * v1 + v2 * v2), "slip1": z, "slip2": z, "slip": z, "Tt": Tt, "rake": rake,
+            "fric": Tt / Tn, "a": a, "b": b, "dc": dc, "f0": f0, "shear_loading": sl, "normal_loading": nl,
+            "P": z, "dPdt0": z, "rake0": rake}
+
+
+def init_members(ens, cols_per_member, V0, Vpl, mu, Cs):
+    """Steady-state initial condition for all members from init-file columns
+    (B, n, 10)."""
+    cols = np.asarray(cols_per_member, dtype=np.float64)
+    per = [steady_state_fields(cols[m], V0, Vpl) for m in range(cols.shape[0])]
+    for k in per[0]:
+        ens.set(k, np.stack([p[k] for p in per]))
     ens.set_params(V0, Vpl, mu / (2.0 * Cs))
+
+
+def load_member(ens, m, cols, V0, Vpl, h0=1e-3, t_end=0.0):
+    """Admit one member into slot m with its init-file columns (n, 10)."""
+    for k, v in steady_state_fields(cols, V0, Vpl).items():
+        ens.set_member(m, k, v)
+    ens.reset_member(m, h0=h0, t_end=t_end, active=True)
