@@ -164,6 +164,55 @@ int pq3d_ens_run(void* ens, long long nsteps, int graph_steps, int async);
 int pq3d_ens_sync(void* ens);
 int pq3d_ens_query(void* ens, int* done);
 
+/* ------------------------------------------------------------------------
+ * 6. H-matrix: batched Green's-function sub-blocks, ACA, device product
+ *
+ * Replaces the block/ACA layer of src/Hmatrix.py: the row/column/sub-block
+ * evaluators calc_stressfunc_fromC* / calc_rows* / calc_cols* (:329-658),
+ * ACA_plus_full_traceS/D (:661-1289), ACA_worker (:1292-1344) and
+ * blocks_process_MVM(_gpu) (:1773-1994, caller src/QDsim.py:1153-1160).
+ * ---------------------------------------------------------------------- */
+#define PQ3D_MAT_NONE 3 /* pq3d_dense_create without matrices (H-matrix attached) */
+
+/* device-resident geometry: nsrc source TDs (P* nsrc x 3), nrcv receivers
+ * (xg nrcv x 3, ev nrcv x 9 frames [e1, e2, n]) */
+int pq3d_gfsrc_create(void** src, const double* P1, const double* P2, const double* P3, long long nsrc,
+                      const double* xg, const double* ev, long long nrcv, double mu, double lam, int halfspace);
+int pq3d_gfsrc_destroy(void* src);
+/* nseg sub-blocks; seg[4k..4k+3] = rcv_off, m, src_off, n into rcv_idx /
+ * src_idx (element ids); out (host) receives per segment 6 x n x m FP64
+ * ([A1s..Bd][column][row]), segments back to back */
+int pq3d_gf_segments(void* src, const int* rcv_idx, long long nr_idx, const int* src_idx, long long ns_idx,
+                     long long nseg, const long long* seg, double* out);
+/* ACA of the six matrices of nb blocks (rows row_idx[row_off[b]:row_off[b+1]],
+ * columns likewise), relative tolerance eps, max_rank <= 0 for min(m, n);
+ * ranks: nb x 6 out; *result holds the factors */
+int pq3d_aca_run(void* src, long long nb, const long long* row_off, const int* row_idx, const long long* col_off,
+                 const int* col_idx, double eps, int max_rank, unsigned long long seed, long long* ranks,
+                 void** result);
+/* stats8: sweeps, launches, pairs evaluated, GPU s, host s, max rank, unconverged, total s */
+int pq3d_aca_stats(void* result, double* stats8);
+/* U (m x k column-major) and V (k x n row-major) per block and matrix A1s..Bd, back to back */
+int pq3d_aca_fetch(void* result, double* U, double* V);
+int pq3d_aca_free(void* result);
+
+/* device H-matrix from leaves (kind 0 dense: six m x n column-major FP64;
+ * kind 1 low-rank: ranks[6b+c], U m x k column-major, V k x n row-major);
+ * perm: cluster position -> element id; factor_type PQ3D_MAT_F32 or _BF16 */
+int pq3d_hmat_create(void** hm, long long n, const int* perm, long long nb, const long long* row_off,
+                     const int* row_idx, const long long* col_off, const int* col_idx, const int* kind,
+                     const long long* ranks, const double* dense, const double* U, const double* V,
+                     int factor_type);
+int pq3d_hmat_destroy(void* hm);
+/* info8: dense bytes, factor bytes, bytes per product, same with fix_tn,
+ * segments, row items, column-slab items, low-rank blocks */
+int pq3d_hmat_info(void* hm, double* info8);
+/* y (3 x n) = (A1s x1 + A1d x2, A2s x1 + A2d x2, Bs x1 + Bd x2) (host arrays) */
+int pq3d_hmat_mvm(void* hm, const double* x1, const double* x2, int fix_tn, double* y);
+int pq3d_hmat_time_mvm(void* hm, int reps, int fix_tn, float* ms_per_product);
+/* the dense simulation's traction rates come from hm from now on */
+int pq3d_dense_attach_hmatrix(void* sim, void* hm);
+
 #ifdef __cplusplus
 }
 #endif

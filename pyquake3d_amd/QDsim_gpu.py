@@ -101,8 +101,14 @@ class DenseSim:
         ptrs = (ctypes.c_void_p * 6)()
         ld = ctypes.c_longlong()
         check(L.pq3d_dense_matrix_ptrs(h, ptrs, ctypes.byref(ld)), "matrix_ptrs")
-        self.mat_ptrs = [int(p) for p in ptrs]
+        self.mat_ptrs = [int(p) if p else 0 for p in ptrs]
         self.ld = ld.value
+
+    def attach_hmatrix(self, hmat):
+        """Take the traction rates from a ``Hmatrix.DeviceHmatrix`` (create the
+        simulation with ``MAT_NONE`` so that no dense matrices are allocated)."""
+        check(_lib.lib().pq3d_dense_attach_hmatrix(self.h, hmat.h), "attach_hmatrix")
+        self._hmat = hmat  # must outlive the simulation's use of it
 
     def set(self, name, arr):
         a = np.ascontiguousarray(np.broadcast_to(np.asarray(arr, dtype=np.float64), (self.n,)))

@@ -16,7 +16,7 @@ _dp = ctypes.POINTER(ctypes.c_double)
 _vp = ctypes.c_void_p
 _i64p = ctypes.POINTER(ctypes.c_longlong)
 
-MAT_F64, MAT_F32, MAT_BF16 = 0, 1, 2
+MAT_F64, MAT_F32, MAT_BF16, MAT_NONE = 0, 1, 2, 3
 _lib = None
 
 

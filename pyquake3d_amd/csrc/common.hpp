@@ -43,7 +43,8 @@ struct Scratch {
 int scratch_get(Scratch& s, size_t bytes, void** out);
 
 // Matrix storage types of the dense traction operators.
-enum MatType : int { MAT_F64 = 0, MAT_F32 = 1, MAT_BF16 = 2 };
-inline size_t mat_elem_bytes(int t) { return t == MAT_F64 ? 8 : (t == MAT_F32 ? 4 : 2); }
+// MAT_NONE: no dense matrices (the traction rates come from an attached H-matrix).
+enum MatType : int { MAT_F64 = 0, MAT_F32 = 1, MAT_BF16 = 2, MAT_NONE = 3 };
+inline size_t mat_elem_bytes(int t) { return t == MAT_F64 ? 8 : (t == MAT_F32 ? 4 : (t == MAT_BF16 ? 2 : 0)); }
 
 }  // namespace pq3d

@@ -1,0 +1,635 @@
+// Device-resident H-matrix of the six traction matrices and its product
+// (the reference's BlockTree.blocks_process_MVM, src/Hmatrix.py:1773-1866,
+// called three times per step pair-wise at src/QDsim.py:1153-1160).
+//
+// Layout.  Rows and columns are in the cluster order (depth-first leaves of
+// the cluster tree, so every block is a contiguous [r0, r0+m) x [c0, c0+n)
+// range).  The rows are cut into segments of at most 64 at every block row
+// boundary; a block contributes one item per segment it covers:
+//   * dense leaf item: FP32 [matrix 6][column n][row ms] -- a wave reads 64
+//     consecutive floats per load;
+//   * low-rank item: per matrix the segment's rows of U, [rank k][row ms], in
+//     the factor type (FP32 or BF16).
+// Low-rank V factors are stored per (block, matrix) as slabs of VEC rows,
+// [slab][column][VEC] (VEC = 16 bytes of the factor type), so a lane loads
+// one 16-byte vector per column.
+//
+// Product y = (A1s x1 + A1d x2, A2s x1 + A2d x2, Bs x1 + Bd x2), four launches:
+//   1. gather: x in cluster order (FP32);
+//   2. phase A: t = V x per (block, matrix, slab) -- one wave, FP32 sums;
+//   3. phase B: per item, partial rows = dense * x or U * t -- one wave, lanes
+//      (row, column group), FP32 sums reduced across groups;
+//   4. reduce: per row, the items of its segment summed in FP64 in a fixed
+//      order (deterministic) and scattered to element order.
+// With fix_tn the normal-traction matrices Bs, Bd are skipped (the step then
+// uses the normal loading alone, QDsim_gpu.py:948-953).
+#include "common.hpp"
+
+#include <algorithm>
+#include <cstdint>
+#include <vector>
+
+namespace pq3d {
+
+struct AItem {
+    long long v_off;  // first element of the slab (layout [column][VEC])
+    int n, c0;        // columns, first cluster-order column
+    int src;          // 0: x1 (strike matrices), 1: x2 (dip matrices)
+    int t_off;        // first of the VEC outputs in t
+    int comp;         // matrix 0..5
+    int pad;
+};
+
+struct BItem {
+    long long off[6];  // dense: off[0] = data [6][n][ms]; low-rank: U rows [k][ms] per matrix
+    int t_off[6];      // low-rank: t offsets per matrix
+    int k[6];          // low-rank ranks
+    int c0, n;         // dense: first cluster-order column, column count
+    int s0, ms;        // segment rows [s0, s0+ms)
+    int slot;          // partial sums at slot: [3][ms]
+    int type;          // 0 dense, 1 low-rank
+};
+
+struct Seg {
+    int s0, ms, item0, item1;
+    int slot0, pad;
+};
+
+struct HMat {
+    long long n = 0;
+    int ftype = MAT_F32;
+    int vec = 4;
+    int* perm = nullptr;      // cluster position -> element id
+    int* row_seg = nullptr;   // cluster position -> segment
+    AItem* a_items = nullptr;
+    long long na = 0;
+    BItem* b_items = nullptr;
+    long long nbi = 0;
+    Seg* segs = nullptr;
+    long long nseg = 0;
+    float* ddata = nullptr;   // dense leaves
+    void* udata = nullptr;    // U rows (ftype)
+    void* vdata = nullptr;    // V slabs (ftype)
+    float* xp = nullptr;      // 2 x n, cluster order
+    float* t = nullptr;
+    float* partial = nullptr;
+    double* xin = nullptr;    // 2 x n, element order (host-call inputs)
+    double* yout = nullptr;   // 3 x n, element order (host-call outputs)
+    long long nt = 0, npart = 0;
+    double bytes_dense = 0, bytes_u = 0, bytes_v = 0, bytes_dense_b = 0, bytes_u_b = 0, bytes_v_b = 0;
+    long long lowrank_blocks = 0;
+};
+
+__device__ __forceinline__ float bf16f(unsigned int b) { return __uint_as_float(b << 16); }
+
+__device__ __forceinline__ float wave_sum(float v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+
+__global__ __launch_bounds__(256) void hmat_gather_kernel(const int* __restrict__ perm, long long n,
+                                                          const double* __restrict__ V1,
+                                                          const double* __restrict__ V2,
+                                                          const double* __restrict__ c0,
+                                                          const double* __restrict__ s0, double Vpl,
+                                                          float* __restrict__ xp) {
+    const long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= n) return;
+    const int e = perm[p];
+    double u1 = V1[e], u2 = V2[e];
+    if (c0) {
+        u1 = __dsub_rn(u1, __dmul_rn(Vpl, c0[e]));
+        u2 = __dsub_rn(u2, __dmul_rn(Vpl, s0[e]));
+    }
+    xp[p] = (float)u1;
+    xp[n + p] = (float)u2;
+}
+
+template <class T> struct Slab;
+template <> struct Slab<float> {
+    static constexpr int VEC = 4;
+    __device__ static void load(const float* base, long long j, float* o) {
+        const float4 a = *reinterpret_cast<const float4*>(base + 4 * j);
+        o[0] = a.x;
+        o[1] = a.y;
+        o[2] = a.z;
+        o[3] = a.w;
+    }
+    __device__ static float one(const float* p, long long i) { return p[i]; }
+};
+template <> struct Slab<unsigned short> {
+    static constexpr int VEC = 8;
+    __device__ static void load(const unsigned short* base, long long j, float* o) {
+        const uint4 a = *reinterpret_cast<const uint4*>(base + 8 * j);
+        const unsigned int w[4] = {a.x, a.y, a.z, a.w};
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            o[2 * k] = bf16f(w[k] & 0xffffu);
+            o[2 * k + 1] = bf16f(w[k] >> 16);
+        }
+    }
+    __device__ static float one(const unsigned short* p, long long i) { return bf16f(p[i]); }
+};
+
+// phase A: one wave per (block, matrix, slab); t[t_off + e] = sum_j V[e][j] x[c0 + j]
+template <class T>
+__global__ __launch_bounds__(256) void hmat_phase_a_kernel(const AItem* __restrict__ items, long long na,
+                                                           const T* __restrict__ vdata,
+                                                           const float* __restrict__ xp, long long n,
+                                                           float* __restrict__ t, int fix_tn) {
+    constexpr int VEC = Slab<T>::VEC;
+    const long long w = (long long)blockIdx.x * 4 + (threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
+    if (w >= na) return;  // whole wave
+    const AItem it = items[w];
+    if (fix_tn && (it.comp == 2 || it.comp == 5)) return;
+    const T* base = vdata + it.v_off;
+    const float* x = xp + (long long)it.src * n + it.c0;
+    float acc[VEC];
+#pragma unroll
+    for (int e = 0; e < VEC; ++e) acc[e] = 0.f;
+    int j = lane;
+    for (; j + 64 < it.n; j += 128) {
+        float v0[VEC], v1[VEC];
+        Slab<T>::load(base, j, v0);
+        Slab<T>::load(base, j + 64, v1);
+        const float x0 = x[j], x1 = x[j + 64];
+#pragma unroll
+        for (int e = 0; e < VEC; ++e) acc[e] = fmaf(v0[e], x0, fmaf(v1[e], x1, acc[e]));
+    }
+    if (j < it.n) {
+        float v0[VEC];
+        Slab<T>::load(base, j, v0);
+        const float x0 = x[j];
+#pragma unroll
+        for (int e = 0; e < VEC; ++e) acc[e] = fmaf(v0[e], x0, acc[e]);
+    }
+    float mine = 0.f;
+#pragma unroll
+    for (int e = 0; e < VEC; ++e) {
+        const float s = wave_sum(acc[e]);
+        if (lane == e) mine = s;
+    }
+    if (lane < VEC) t[it.t_off + lane] = mine;
+}
+
+// phase B: one wave per item; lane = g * mp + i (row i of the segment, column group g)
+template <class T>
+__global__ __launch_bounds__(256) void hmat_phase_b_kernel(const BItem* __restrict__ items, long long nitems,
+                                                           const float* __restrict__ ddata,
+                                                           const T* __restrict__ udata,
+                                                           const float* __restrict__ xp, long long n,
+                                                           const float* __restrict__ t,
+                                                           float* __restrict__ partial, int fix_tn) {
+    const long long w = (long long)blockIdx.x * 4 + (threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
+    if (w >= nitems) return;  // whole wave
+    const BItem it = items[w];
+    const int ms = it.ms;
+    const int mp = ms > 32 ? 64 : (ms > 16 ? 32 : (ms > 8 ? 16 : (ms > 4 ? 8 : (ms > 2 ? 4 : (ms > 1 ? 2 : 1)))));
+    const int G = 64 / mp;
+    const int i = lane & (mp - 1), g = lane / mp;
+    const bool row_ok = i < ms;
+    float acc0 = 0.f, acc1 = 0.f, acc2 = 0.f;
+    if (it.type == 0) {
+        const float* D = ddata + it.off[0];
+        const float* x1 = xp + it.c0;
+        const float* x2 = xp + n + it.c0;
+        const long long cs = (long long)it.n * ms;  // matrix stride
+        for (int j = g; j < it.n; j += G) {
+            if (!row_ok) continue;
+            const float a = x1[j], b = x2[j];
+            const float* col = D + (long long)j * ms + i;
+            acc0 = fmaf(col[0], a, acc0);
+            acc1 = fmaf(col[cs], a, acc1);
+            acc0 = fmaf(col[3 * cs], b, acc0);
+            acc1 = fmaf(col[4 * cs], b, acc1);
+            if (!fix_tn) {
+                acc2 = fmaf(col[2 * cs], a, acc2);
+                acc2 = fmaf(col[5 * cs], b, acc2);
+            }
+        }
+    } else {
+#pragma unroll
+        for (int c = 0; c < 6; ++c) {
+            if (fix_tn && (c == 2 || c == 5)) continue;
+            const int k = it.k[c];
+            const T* U = udata + it.off[c];
+            const float* tt = t + it.t_off[c];
+            float s = 0.f;
+            if (row_ok)
+                for (int l = g; l < k; l += G) s = fmaf(Slab<T>::one(U, (long long)l * ms + i), tt[l], s);
+            if (c % 3 == 0) acc0 += s;
+            else if (c % 3 == 1) acc1 += s;
+            else acc2 += s;
+        }
+    }
+    for (int o = mp; o < 64; o <<= 1) {
+        acc0 += __shfl_xor(acc0, o, 64);
+        acc1 += __shfl_xor(acc1, o, 64);
+        acc2 += __shfl_xor(acc2, o, 64);
+    }
+    if (g == 0 && row_ok) {
+        float* P = partial + it.slot;
+        P[i] = acc0;
+        P[ms + i] = acc1;
+        P[2 * ms + i] = acc2;
+    }
+}
+
+// reduce: per cluster-order row, sum its segment's items (FP64, fixed order)
+// and scatter to element order: out[q * n + perm[p]]
+__global__ __launch_bounds__(256) void hmat_reduce_kernel(const Seg* __restrict__ segs,
+                                                          const int* __restrict__ row_seg,
+                                                          const int* __restrict__ perm, long long n,
+                                                          const float* __restrict__ partial,
+                                                          double* __restrict__ out) {
+    const long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= n) return;
+    const Seg s = segs[row_seg[p]];
+    const int i = (int)(p - s.s0);
+    double y0 = 0, y1 = 0, y2 = 0;
+    const float* P = partial + s.slot0 + i;
+    const long long stride = 3LL * s.ms;
+    for (int it = s.item0; it < s.item1; ++it, P += stride) {
+        y0 += (double)P[0];
+        y1 += (double)P[s.ms];
+        y2 += (double)P[2 * s.ms];
+    }
+    const int e = perm[p];
+    out[e] = y0;
+    out[n + e] = y1;
+    out[2 * n + e] = y2;
+}
+
+static unsigned short f2bf(float f) {
+    uint32_t u;
+    memcpy(&u, &f, 4);
+    if ((u & 0x7fffffffu) > 0x7f800000u) return (unsigned short)((u >> 16) | 0x40);  // NaN stays NaN
+    u += 0x7fffu + ((u >> 16) & 1u);
+    return (unsigned short)(u >> 16);
+}
+
+template <class T> static T conv(double x);
+template <> float conv<float>(double x) { return (float)x; }
+template <> unsigned short conv<unsigned short>(double x) { return f2bf((float)x); }
+
+template <class T>
+static void launch_mvm_t(HMat* h, const double* V1, const double* V2, const double* c0, const double* s0,
+                         double Vpl, int fix_tn, double* out, hipStream_t st) {
+    const long long n = h->n;
+    hipLaunchKernelGGL(hmat_gather_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, h->perm, n, V1, V2,
+                       c0, s0, Vpl, h->xp);
+    if (h->na)
+        hipLaunchKernelGGL(hmat_phase_a_kernel<T>, dim3((unsigned)((h->na + 3) / 4)), dim3(256), 0, st, h->a_items,
+                           h->na, (const T*)h->vdata, h->xp, n, h->t, fix_tn);
+    if (h->nbi)
+        hipLaunchKernelGGL(hmat_phase_b_kernel<T>, dim3((unsigned)((h->nbi + 3) / 4)), dim3(256), 0, st,
+                           h->b_items, h->nbi, h->ddata, (const T*)h->udata, h->xp, n, h->t, h->partial, fix_tn);
+    hipLaunchKernelGGL(hmat_reduce_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, h->segs,
+                       h->row_seg, h->perm, n, h->partial, out);
+}
+
+// Traction rates of one step into out (3 x n, element order): x1 = V1 - Vpl
+// cos(rake0), x2 = V2 - Vpl sin(rake0) (c0/s0 null: x = V as given).
+void hmat_launch(void* hm, const double* V1, const double* V2, const double* c0, const double* s0, double Vpl,
+                 int fix_tn, double* out, hipStream_t st) {
+    HMat* h = (HMat*)hm;
+    if (h->ftype == MAT_BF16) launch_mvm_t<unsigned short>(h, V1, V2, c0, s0, Vpl, fix_tn, out, st);
+    else launch_mvm_t<float>(h, V1, V2, c0, s0, Vpl, fix_tn, out, st);
+}
+
+long long hmat_n(void* hm) { return ((HMat*)hm)->n; }
+
+template <class T>
+static int pack(HMat* h, long long n, const int* perm, long long nb, const long long* row_off, const int* row_idx,
+                const long long* col_off, const int* col_idx, const int* kind, const long long* ranks,
+                const double* dense, const double* U, const double* V) {
+    constexpr int VEC = 16 / sizeof(T);
+    h->vec = VEC;
+    std::vector<int> inv(n, -1);
+    for (long long p = 0; p < n; ++p) {
+        PQ_CHECK(perm[p] >= 0 && perm[p] < n && inv[perm[p]] < 0, "pq3d_hmat_create: perm is not a permutation");
+        inv[perm[p]] = (int)p;
+    }
+    std::vector<int> r0(nb), c0(nb);
+    std::vector<int> bnd;
+    bnd.push_back(0);
+    bnd.push_back((int)n);
+    double cover = 0;
+    for (long long b = 0; b < nb; ++b) {
+        const long long m = row_off[b + 1] - row_off[b], nc = col_off[b + 1] - col_off[b];
+        PQ_CHECK(m > 0 && nc > 0, "pq3d_hmat_create: empty block %lld", b);
+        int lo = INT32_MAX, hi = -1;
+        for (long long i = row_off[b]; i < row_off[b + 1]; ++i) {
+            PQ_CHECK(row_idx[i] >= 0 && row_idx[i] < n, "pq3d_hmat_create: row id out of range");
+            lo = std::min(lo, inv[row_idx[i]]);
+            hi = std::max(hi, inv[row_idx[i]]);
+        }
+        PQ_CHECK(hi - lo + 1 == m, "pq3d_hmat_create: rows of block %lld are not a cluster range", b);
+        r0[b] = lo;
+        lo = INT32_MAX;
+        hi = -1;
+        for (long long j = col_off[b]; j < col_off[b + 1]; ++j) {
+            PQ_CHECK(col_idx[j] >= 0 && col_idx[j] < n, "pq3d_hmat_create: column id out of range");
+            lo = std::min(lo, inv[col_idx[j]]);
+            hi = std::max(hi, inv[col_idx[j]]);
+        }
+        PQ_CHECK(hi - lo + 1 == nc, "pq3d_hmat_create: columns of block %lld are not a cluster range", b);
+        c0[b] = lo;
+        bnd.push_back(r0[b]);
+        bnd.push_back(r0[b] + (int)m);
+        cover += (double)m * (double)nc;
+        for (int c = 0; c < 6; ++c)
+            PQ_CHECK(kind[b] == 0 || (ranks[6 * b + c] >= 0 && ranks[6 * b + c] <= std::min(m, nc)),
+                     "pq3d_hmat_create: bad rank in block %lld", b);
+    }
+    PQ_CHECK(cover == (double)n * (double)n, "pq3d_hmat_create: blocks cover %.0f entries, not n^2", cover);
+    std::sort(bnd.begin(), bnd.end());
+    bnd.erase(std::unique(bnd.begin(), bnd.end()), bnd.end());
+    std::vector<Seg> segs;
+    for (size_t k = 0; k + 1 < bnd.size(); ++k) {
+        const int a = bnd[k], len = bnd[k + 1] - bnd[k];
+        const int pieces = (len + 63) / 64;
+        for (int q = 0; q < pieces; ++q) {
+            const int s = a + (int)((long long)len * q / pieces), e = a + (int)((long long)len * (q + 1) / pieces);
+            segs.push_back(Seg{s, e - s, 0, 0, 0, 0});
+        }
+    }
+    std::vector<int> seg_start(segs.size());
+    for (size_t s = 0; s < segs.size(); ++s) seg_start[s] = segs[s].s0;
+    // items: per segment, the blocks covering it in block order
+    std::vector<std::vector<int>> seg_blocks(segs.size());
+    for (long long b = 0; b < nb; ++b) {
+        const long long m = row_off[b + 1] - row_off[b];
+        size_t s = std::lower_bound(seg_start.begin(), seg_start.end(), r0[b]) - seg_start.begin();
+        for (; s < segs.size() && segs[s].s0 < r0[b] + m; ++s) seg_blocks[s].push_back((int)b);
+    }
+    // offsets of every block's inputs
+    std::vector<long long> doff(nb, -1), uoff(nb * 6, 0), voff(nb * 6, 0);
+    {
+        long long d = 0, u = 0, v = 0;
+        for (long long b = 0; b < nb; ++b) {
+            const long long m = row_off[b + 1] - row_off[b], nc = col_off[b + 1] - col_off[b];
+            if (kind[b] == 0) {
+                doff[b] = d;
+                d += 6 * m * nc;
+            } else {
+                for (int c = 0; c < 6; ++c) {
+                    uoff[6 * b + c] = u;
+                    voff[6 * b + c] = v;
+                    u += ranks[6 * b + c] * m;
+                    v += ranks[6 * b + c] * nc;
+                }
+            }
+        }
+    }
+    // block-local index of each cluster-order offset
+    std::vector<long long> rinv_off(nb + 1, 0), cinv_off(nb + 1, 0);
+    for (long long b = 0; b < nb; ++b) {
+        rinv_off[b + 1] = rinv_off[b] + (row_off[b + 1] - row_off[b]);
+        cinv_off[b + 1] = cinv_off[b] + (col_off[b + 1] - col_off[b]);
+    }
+    std::vector<int> rinv(rinv_off[nb]), cinv(cinv_off[nb]);
+    for (long long b = 0; b < nb; ++b) {
+        for (long long i = row_off[b]; i < row_off[b + 1]; ++i)
+            rinv[rinv_off[b] + inv[row_idx[i]] - r0[b]] = (int)(i - row_off[b]);
+        for (long long j = col_off[b]; j < col_off[b + 1]; ++j)
+            cinv[cinv_off[b] + inv[col_idx[j]] - c0[b]] = (int)(j - col_off[b]);
+    }
+    // t offsets and V slabs (phase A)
+    std::vector<int> toff(nb * 6, 0);
+    std::vector<AItem> aitems;
+    std::vector<T> vpack;
+    long long tcount = 0;
+    h->bytes_v = h->bytes_v_b = 0;
+    h->lowrank_blocks = 0;
+    for (long long b = 0; b < nb; ++b) {
+        if (kind[b] == 0) continue;
+        ++h->lowrank_blocks;
+        const long long m = row_off[b + 1] - row_off[b], nc = col_off[b + 1] - col_off[b];
+        for (int c = 0; c < 6; ++c) {
+            const long long k = ranks[6 * b + c];
+            toff[6 * b + c] = (int)tcount;
+            if (k == 0) continue;
+            const long long kp = (k + VEC - 1) / VEC * VEC;
+            const double* Vb = V + voff[6 * b + c];
+            const long long base = (long long)vpack.size();
+            vpack.resize((size_t)(base + kp * nc));
+            for (long long lb = 0; lb < kp / VEC; ++lb) {
+                for (long long jp = 0; jp < nc; ++jp) {
+                    const int j = cinv[cinv_off[b] + jp];
+                    for (int e = 0; e < VEC; ++e) {
+                        const long long l = lb * VEC + e;
+                        vpack[(size_t)(base + (lb * nc + jp) * VEC + e)] = conv<T>(l < k ? Vb[l * nc + j] : 0.0);
+                    }
+                }
+                aitems.push_back(AItem{base + lb * nc * VEC, (int)nc, c0[b], c / 3, (int)(tcount + lb * VEC), c, 0});
+            }
+            const double bytes = (double)kp * nc * sizeof(T);
+            h->bytes_v += bytes;
+            if (c != 2 && c != 5) h->bytes_v_b += bytes;
+            tcount += kp;
+        }
+        (void)m;
+    }
+    // phase B items
+    std::vector<BItem> bitems;
+    std::vector<float> dpack;
+    std::vector<T> upack;
+    long long slot = 0;
+    h->bytes_dense = h->bytes_dense_b = h->bytes_u = h->bytes_u_b = 0;
+    for (size_t s = 0; s < segs.size(); ++s) {
+        Seg& sg = segs[s];
+        sg.item0 = (int)bitems.size();
+        sg.slot0 = (int)slot;
+        for (int b : seg_blocks[s]) {
+            const long long m = row_off[b + 1] - row_off[b], nc = col_off[b + 1] - col_off[b];
+            const int ms = sg.ms;
+            const int lr0 = sg.s0 - r0[b];  // first block-local cluster row of the segment
+            BItem it;
+            memset(&it, 0, sizeof(it));
+            it.s0 = sg.s0;
+            it.ms = ms;
+            it.slot = (int)slot;
+            it.type = kind[b];
+            if (kind[b] == 0) {
+                it.c0 = c0[b];
+                it.n = (int)nc;
+                const long long base = (long long)dpack.size();
+                it.off[0] = base;
+                dpack.resize((size_t)(base + 6 * nc * ms));
+                const double* Db = dense + doff[b];
+                for (int c = 0; c < 6; ++c)
+                    for (long long jp = 0; jp < nc; ++jp) {
+                        const int j = cinv[cinv_off[b] + jp];
+                        for (int ip = 0; ip < ms; ++ip) {
+                            const int i = rinv[rinv_off[b] + lr0 + ip];
+                            dpack[(size_t)(base + ((long long)c * nc + jp) * ms + ip)] =
+                                (float)Db[(long long)c * m * nc + (long long)j * m + i];
+                        }
+                    }
+                const double bytes = 6.0 * nc * ms * 4;
+                h->bytes_dense += bytes;
+                h->bytes_dense_b += bytes * 4 / 6;
+            } else {
+                for (int c = 0; c < 6; ++c) {
+                    const long long k = ranks[6 * b + c];
+                    it.k[c] = (int)k;
+                    it.t_off[c] = toff[6 * b + c];
+                    const long long base = (long long)upack.size();
+                    it.off[c] = base;
+                    if (k == 0) continue;
+                    upack.resize((size_t)(base + k * ms));
+                    const double* Ub = U + uoff[6 * b + c];
+                    for (long long l = 0; l < k; ++l)
+                        for (int ip = 0; ip < ms; ++ip) {
+                            const int i = rinv[rinv_off[b] + lr0 + ip];
+                            upack[(size_t)(base + l * ms + ip)] = conv<T>(Ub[l * m + i]);
+                        }
+                    const double bytes = (double)k * ms * sizeof(T);
+                    h->bytes_u += bytes;
+                    if (c != 2 && c != 5) h->bytes_u_b += bytes;
+                }
+            }
+            bitems.push_back(it);
+            slot += 3 * ms;
+        }
+        sg.item1 = (int)bitems.size();
+    }
+    PQ_CHECK(slot < INT32_MAX && tcount < INT32_MAX, "pq3d_hmat_create: H-matrix too large for 32-bit offsets");
+    std::vector<int> row_seg(n);
+    for (size_t s = 0; s < segs.size(); ++s)
+        for (int p = segs[s].s0; p < segs[s].s0 + segs[s].ms; ++p) row_seg[p] = (int)s;
+    // upload
+    h->n = n;
+    h->na = (long long)aitems.size();
+    h->nbi = (long long)bitems.size();
+    h->nseg = (long long)segs.size();
+    h->nt = tcount;
+    h->npart = slot;
+    auto up = [&](void** dst, const void* src, size_t bytes) -> int {
+        PQ_HIP(hipMalloc(dst, bytes ? bytes : 16));
+        if (bytes) PQ_HIP(hipMemcpy(*dst, src, bytes, hipMemcpyHostToDevice));
+        return 0;
+    };
+    if (up((void**)&h->perm, perm, n * sizeof(int))) return -1;
+    if (up((void**)&h->row_seg, row_seg.data(), n * sizeof(int))) return -1;
+    if (up((void**)&h->a_items, aitems.data(), aitems.size() * sizeof(AItem))) return -1;
+    if (up((void**)&h->b_items, bitems.data(), bitems.size() * sizeof(BItem))) return -1;
+    if (up((void**)&h->segs, segs.data(), segs.size() * sizeof(Seg))) return -1;
+    if (up((void**)&h->ddata, dpack.data(), dpack.size() * sizeof(float))) return -1;
+    if (up(&h->udata, upack.data(), upack.size() * sizeof(T))) return -1;
+    if (up(&h->vdata, vpack.data(), vpack.size() * sizeof(T))) return -1;
+    PQ_HIP(hipMalloc((void**)&h->xp, 2 * n * sizeof(float)));
+    PQ_HIP(hipMalloc((void**)&h->t, (tcount + 1) * sizeof(float)));
+    PQ_HIP(hipMalloc((void**)&h->partial, (slot + 1) * sizeof(float)));
+    PQ_HIP(hipMalloc((void**)&h->xin, 2 * n * sizeof(double)));
+    PQ_HIP(hipMalloc((void**)&h->yout, 3 * n * sizeof(double)));
+    PQ_HIP(hipMemset(h->xin, 0, 2 * n * sizeof(double)));
+    PQ_HIP(hipMemset(h->t, 0, (tcount + 1) * sizeof(float)));
+    return 0;
+}
+
+static void hmat_free(HMat* h) {
+    void* ptrs[] = {h->perm, h->row_seg, h->a_items, h->b_items, h->segs, h->ddata, h->udata,
+                    h->vdata, h->xp, h->t, h->partial, h->xin, h->yout};
+    for (void* p : ptrs)
+        if (p) (void)hipFree(p);
+    delete h;
+}
+
+}  // namespace pq3d
+
+using namespace pq3d;
+
+extern "C" {
+
+// Build the device H-matrix.  perm (n): cluster position -> element id.
+// Block b: rows row_idx[row_off[b]:row_off[b+1]], columns col_idx[...] (element
+// ids; each must be a contiguous range of the cluster order); kind[b] = 0
+// dense (six m x n column-major FP64 matrices in `dense`, blocks back to
+// back), 1 low-rank (ranks[6b+c]; U m x k column-major in `U`, V k x n
+// row-major in `V`, matrices A1s..Bd, blocks back to back).  factor_type:
+// PQ3D_MAT_F32 or PQ3D_MAT_BF16 for the low-rank factors (dense leaves FP32).
+int pq3d_hmat_create(void** out, long long n, const int* perm, long long nb, const long long* row_off,
+                     const int* row_idx, const long long* col_off, const int* col_idx, const int* kind,
+                     const long long* ranks, const double* dense, const double* U, const double* V,
+                     int factor_type) {
+    clear_error();
+    PQ_CHECK(n > 0 && nb > 0, "pq3d_hmat_create: empty H-matrix");
+    PQ_CHECK(factor_type == MAT_F32 || factor_type == MAT_BF16, "pq3d_hmat_create: factor type must be f32 or bf16");
+    HMat* h = new HMat();
+    h->ftype = factor_type;
+    int rc = factor_type == MAT_BF16
+                 ? pack<unsigned short>(h, n, perm, nb, row_off, row_idx, col_off, col_idx, kind, ranks, dense, U, V)
+                 : pack<float>(h, n, perm, nb, row_off, row_idx, col_off, col_idx, kind, ranks, dense, U, V);
+    if (rc) {
+        hmat_free(h);
+        return rc;
+    }
+    *out = h;
+    return 0;
+}
+
+int pq3d_hmat_destroy(void* hm) {
+    clear_error();
+    if (hm) hmat_free((HMat*)hm);
+    return 0;
+}
+
+// info8: dense bytes, factor bytes, bytes read per product, same with fix_tn,
+// segments, phase-B items, phase-A items, low-rank blocks
+int pq3d_hmat_info(void* hm, double* info8) {
+    clear_error();
+    HMat* h = (HMat*)hm;
+    PQ_CHECK(h, "pq3d_hmat_info: null");
+    info8[0] = h->bytes_dense;
+    info8[1] = h->bytes_u + h->bytes_v;
+    info8[2] = h->bytes_dense + h->bytes_u + h->bytes_v;
+    info8[3] = h->bytes_dense_b + h->bytes_u_b + h->bytes_v_b;
+    info8[4] = (double)h->nseg;
+    info8[5] = (double)h->nbi;
+    info8[6] = (double)h->na;
+    info8[7] = (double)h->lowrank_blocks;
+    return 0;
+}
+
+// y (host, 3 x n) = (A1s x1 + A1d x2, A2s x1 + A2d x2, Bs x1 + Bd x2); x1, x2 host
+int pq3d_hmat_mvm(void* hm, const double* x1, const double* x2, int fix_tn, double* y) {
+    clear_error();
+    HMat* h = (HMat*)hm;
+    PQ_CHECK(h, "pq3d_hmat_mvm: null");
+    const long long n = h->n;
+    PQ_HIP(hipMemcpy(h->xin, x1, n * sizeof(double), hipMemcpyHostToDevice));
+    PQ_HIP(hipMemcpy(h->xin + n, x2, n * sizeof(double), hipMemcpyHostToDevice));
+    hmat_launch(h, h->xin, h->xin + n, nullptr, nullptr, 0.0, fix_tn, h->yout, 0);
+    PQ_HIP(hipGetLastError());
+    PQ_HIP(hipMemcpy(y, h->yout, 3 * n * sizeof(double), hipMemcpyDeviceToHost));
+    return 0;
+}
+
+// ms per product over `reps` back-to-back products on the last inputs
+int pq3d_hmat_time_mvm(void* hm, int reps, int fix_tn, float* ms_per) {
+    clear_error();
+    HMat* h = (HMat*)hm;
+    PQ_CHECK(h && reps > 0, "pq3d_hmat_time_mvm: bad arguments");
+    const long long n = h->n;
+    hipEvent_t a, b;
+    PQ_HIP(hipEventCreate(&a));
+    PQ_HIP(hipEventCreate(&b));
+    hmat_launch(h, h->xin, h->xin + n, nullptr, nullptr, 0.0, fix_tn, h->yout, 0);
+    PQ_HIP(hipEventRecord(a, 0));
+    for (int r = 0; r < reps; ++r) hmat_launch(h, h->xin, h->xin + n, nullptr, nullptr, 0.0, fix_tn, h->yout, 0);
+    PQ_HIP(hipEventRecord(b, 0));
+    PQ_HIP(hipEventSynchronize(b));
+    float ms = 0;
+    PQ_HIP(hipEventElapsedTime(&ms, a, b));
+    *ms_per = ms / reps;
+    (void)hipEventDestroy(a);
+    (void)hipEventDestroy(b);
+    return 0;
+}
+
+}  // extern "C"

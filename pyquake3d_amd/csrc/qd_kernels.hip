@@ -20,6 +20,7 @@
 //      same kernel.
 #include "common.hpp"
 
+#include <algorithm>
 #include <cstdint>
 #include <vector>
 
@@ -497,7 +498,13 @@ __global__ void pad_zero_kernel(double* p, long long n) {
 // ---------------------------------------------------------------------------
 // Host side: the dense simulation object
 // ---------------------------------------------------------------------------
+// H-matrix traction rates (hmat_mvm.hip): out = 3 x n element-order sums
+void hmat_launch(void* hm, const double* V1, const double* V2, const double* c0, const double* s0, double Vpl,
+                 int fix_tn, double* out, hipStream_t st);
+long long hmat_n(void* hm);
+
 struct DenseSim {
+    void* hmat = nullptr;          // attached H-matrix (replaces the dense matrices)
     long long n = 0, ld = 0;
     int mat_type = MAT_F32;
     void* mats[6] = {nullptr};
@@ -561,6 +568,13 @@ static void launch_matvec_t(DenseSim* s, hipStream_t st) {
 }
 
 static void launch_matvec(DenseSim* s, hipStream_t st) {
+    if (s->hmat) {
+        double* F = s->fields;
+        const long long n = s->n;
+        hmat_launch(s->hmat, F + F_SLIPV1 * n, F + F_SLIPV2 * n, F + F_COSR0 * n, F + F_SINR0 * n, s->Vpl,
+                    s->fix_tn, s->partial, st);
+        return;
+    }
     if (s->mat_type == MAT_F64) launch_matvec_t<double>(s, st);
     else if (s->mat_type == MAT_F32) launch_matvec_t<float>(s, st);
     else launch_matvec_t<unsigned short>(s, st);
@@ -596,13 +610,14 @@ extern "C" {
 int pq3d_dense_create(void** out, long long n, int mat_type, int kc) {
     clear_error();
     PQ_CHECK(n > 0, "pq3d_dense_create: n must be positive");
-    PQ_CHECK(mat_type == MAT_F64 || mat_type == MAT_F32 || mat_type == MAT_BF16, "bad matrix type %d", mat_type);
+    PQ_CHECK(mat_type == MAT_F64 || mat_type == MAT_F32 || mat_type == MAT_BF16 || mat_type == MAT_NONE,
+             "bad matrix type %d", mat_type);
     DenseSim* s = new DenseSim();
     s->n = n;
     s->mat_type = mat_type;
-    const int VN = vec_n(mat_type);
+    const int VN = mat_type == MAT_NONE ? 1 : vec_n(mat_type);
     s->ld = (n + VN - 1) / VN * VN;
-    s->kc = kc > 0 ? kc : 256;
+    s->kc = mat_type == MAT_NONE ? (int)std::min<long long>(n, 1 << 30) : (kc > 0 ? kc : 256);
     s->nchunk = (int)((n + s->kc - 1) / s->kc);
     int rc = configure(s);
     if (rc) {
@@ -610,7 +625,7 @@ int pq3d_dense_create(void** out, long long n, int mat_type, int kc) {
         return rc;
     }
     const size_t mbytes = (size_t)s->ld * (size_t)n * mat_elem_bytes(mat_type);
-    for (int k = 0; k < 6; ++k) {
+    for (int k = 0; k < 6 && mbytes; ++k) {
         hipError_t e = hipMalloc(&s->mats[k], mbytes);
         if (e != hipSuccess) {
             set_error("pq3d_dense_create: matrix allocation of %zu bytes failed: %s", mbytes, hipGetErrorString(e));
@@ -641,6 +656,23 @@ int pq3d_dense_destroy(void* h) {
     (void)hipFree(s->ctrl);
     (void)hipFree(s->d_records);
     delete s;
+    return 0;
+}
+
+// Take the traction rates from an H-matrix (pq3d_hmat_create) instead of the
+// dense matrices; the H-matrix must outlive the simulation's use of it.
+int pq3d_dense_attach_hmatrix(void* h, void* hm) {
+    clear_error();
+    DenseSim* s = (DenseSim*)h;
+    PQ_CHECK(s && hm, "pq3d_dense_attach_hmatrix: null argument");
+    PQ_CHECK(hmat_n(hm) == s->n, "pq3d_dense_attach_hmatrix: H-matrix has n=%lld, simulation n=%lld", hmat_n(hm),
+             s->n);
+    s->hmat = hm;
+    s->nchunk = 1;
+    if (s->graph) {
+        (void)hipGraphExecDestroy(s->graph);
+        s->graph = nullptr;
+    }
     return 0;
 }
 
@@ -747,6 +779,7 @@ __global__ void reduce_partials_kernel(const double* __restrict__ partial, int n
 int pq3d_dense_matvec(void* h) {
     clear_error();
     DenseSim* s = (DenseSim*)h;
+    PQ_CHECK(s->mat_type != MAT_NONE || s->hmat, "dense sim: no matrices and no H-matrix attached");
     launch_matvec(s, 0);
     const long long n = s->n;
     double* F = s->fields;
@@ -762,6 +795,7 @@ int pq3d_dense_matvec(void* h) {
 int pq3d_dense_time_matvec(void* h, int reps, float* ms_per) {
     clear_error();
     DenseSim* s = (DenseSim*)h;
+    PQ_CHECK(s->mat_type != MAT_NONE || s->hmat, "dense sim: no matrices and no H-matrix attached");
     hipEvent_t a, b;
     PQ_HIP(hipEventCreate(&a));
     PQ_HIP(hipEventCreate(&b));
@@ -783,6 +817,7 @@ int pq3d_dense_time_matvec(void* h, int reps, float* ms_per) {
 int pq3d_dense_step(void* h, double dttry, double* h_out, double* dtnext_out, int* nrjct_out) {
     clear_error();
     DenseSim* s = (DenseSim*)h;
+    PQ_CHECK(s->mat_type != MAT_NONE || s->hmat, "dense sim: no matrices and no H-matrix attached");
     PQ_HIP(hipMemcpy(&s->ctrl->dttry, &dttry, sizeof(double), hipMemcpyHostToDevice));
     launch_step(s, 0, nullptr, -1);
     PQ_HIP(hipGetLastError());
@@ -806,6 +841,7 @@ int pq3d_dense_step(void* h, double dttry, double* h_out, double* dtnext_out, in
 int pq3d_dense_run(void* h, long long nsteps, int graph_steps, double* records, int* status) {
     clear_error();
     DenseSim* s = (DenseSim*)h;
+    PQ_CHECK(s->mat_type != MAT_NONE || s->hmat, "dense sim: no matrices and no H-matrix attached");
     if (graph_steps <= 0) graph_steps = 1;
     if (nsteps <= 0) return 0;
     if (records && s->rec_cap < nsteps) {

@@ -63,7 +63,7 @@ class Ensemble:
         ptrs = (ctypes.c_void_p * 6)()
         ld = ctypes.c_longlong()
         check(L.pq3d_ens_matrix_ptrs(h, ptrs, ctypes.byref(ld)), "matrix_ptrs")
-        self.mat_ptrs = [int(p) for p in ptrs]
+        self.mat_ptrs = [int(p) if p else 0 for p in ptrs]
         self.ld = ld.value
 
     # ---- fields

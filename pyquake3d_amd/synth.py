@@ -56,6 +56,55 @@ def planar_fault_mesh(nx, nz, length_km, depth_km, x0_km=None):
     return nodes, np.array(tri, dtype=np.int64)
 
 
+def rough_fault_mesh(nx=300, nz=100, length_km=120.0, depth_km=30.0, amp_km=1.0, wavelength_km=40.0,
+                     hurst=0.8, rough_ratio=1e-3, seed=7):
+    """Nodes (km) and 1-based triangles of a non-planar vertical fault.
+
+    Near-equilateral lattice: nz + 1 node rows from z = -depth to 0, each with
+    nx + 1 nodes spaced length/nx, odd rows shifted by half a spacing; every
+    strip between two rows holds 2 nx triangles, 2 nx nz in all (300 x 100 ->
+    60,000 triangles of sides 400 m and about 360 m).  The fault-normal offset
+    is y = amp sin(2 pi x / wavelength) + h(x, z), with h a seeded self-affine
+    random surface (Fourier amplitudes ~ |k|^-(1 + hurst), random phases)
+    scaled to an RMS height of rough_ratio x length.  Triangles are ordered so
+    that their normals point along +y.
+    """
+    dx = length_km / nx
+    ks = np.arange(nz + 1)
+    js = np.arange(nx + 1)
+    X = -0.5 * length_km + dx * (js[None, :] + 0.5 * (ks[:, None] % 2))
+    Z = np.broadcast_to((-depth_km + depth_km * ks / nz)[:, None], X.shape)
+    rng = np.random.default_rng(seed)
+    kz = np.fft.fftfreq(nz + 1)[:, None]
+    kx = np.fft.fftfreq(nx + 1)[None, :]
+    kk = np.hypot(kx, kz)
+    kk[0, 0] = np.inf
+    spec = kk ** (-(1.0 + hurst)) * np.exp(2j * np.pi * rng.random(kk.shape))
+    h = np.fft.ifft2(spec).real
+    h *= rough_ratio * length_km / np.sqrt(np.mean(h * h))
+    Y = amp_km * np.sin(2.0 * np.pi * X / wavelength_km) + h
+    nodes = np.stack([X.ravel(), Y.ravel(), Z.ravel()], axis=1)
+
+    def nid(k, j):
+        return k * (nx + 1) + j + 1
+
+    tri = []
+    for k in range(nz):
+        for j in range(nx):
+            if k % 2 == 0:   # upper row shifted right
+                tri.append((nid(k, j), nid(k, j + 1), nid(k + 1, j)))
+                tri.append((nid(k + 1, j), nid(k, j + 1), nid(k + 1, j + 1)))
+            else:            # lower row shifted right
+                tri.append((nid(k, j), nid(k + 1, j + 1), nid(k + 1, j)))
+                tri.append((nid(k, j), nid(k, j + 1), nid(k + 1, j + 1)))
+    tri = np.array(tri, dtype=np.int64)
+    p = nodes[tri - 1]
+    nrm = np.cross(p[:, 1] - p[:, 0], p[:, 2] - p[:, 0])
+    flip = nrm[:, 1] < 0
+    tri[flip] = tri[flip][:, [0, 2, 1]]
+    return nodes, tri
+
+
 def write_msh(fname, nodes_km, tri):
     with open(fname, "w") as f:
         f.write("$MeshFormat\n2.2 0 8\n$EndMeshFormat\n$Nodes\n%d\n" % len(nodes_km))
