@@ -22,8 +22,13 @@ Legs
 * cpu_baseline: the NumPy FP64 restatement of the reference's dense step
   (oracle/qdsim_oracle.py: six np.dot per step, BLAS on all host cores) on the
   same 20k FP64 matrices, over a bounded number of steps.
+* hmatrix (separate mesh): 60,000-element rough vertical fault, H-matrix with
+  leaf 64, eta 2, ACA 1e-4, BF16 low-rank factors and FP32 dense leaves: build
+  time, product time against 8 TB/s, error against the exact FP64 dense
+  operator (assembled in HBM, 173 GB) overall and per block class, stepping
+  rate with the H-matrix and with the dense FP64 matrices.
 
-Usage: python bench.py [--gpus 1] [--steps K] [--warmup W]
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-hmat | --hmat-only]
 """
 import argparse
 import json
@@ -195,6 +200,169 @@ def cpu_baseline_assembly(P, xg):
                       f"(strike and dip calls), {threads} processes"}
 
 
+def _bf16(a):
+    """Round to BF16 (nearest even) and back, as the device stores factors."""
+    f = np.ascontiguousarray(a, dtype=np.float32)
+    b = f.view(np.uint32)
+    r = ((b + 0x7FFF + ((b >> 16) & 1)) >> 16).astype(np.uint32) << 16
+    return r.view(np.float32).astype(np.float64)
+
+
+def hmatrix_class_errors(tree, factor_round, rng, per_class=40, sub=128):
+    """Relative Frobenius error of the stored blocks against exact FP64
+    entries on random sub-blocks (<= sub x sub), per block class."""
+    classes = {"dense_leaf": [], "lowrank_m<=256": [], "lowrank_256<m<=2048": [], "lowrank_m>2048": []}
+    for b in tree.blocks_to_process:
+        m = len(b.row_cluster)
+        key = "dense_leaf" if not b.judaca else (
+            "lowrank_m<=256" if m <= 256 else ("lowrank_256<m<=2048" if m <= 2048 else "lowrank_m>2048"))
+        classes[key].append(b)
+    out = {}
+    for key, blocks in classes.items():
+        if not blocks:
+            continue
+        pick = rng.choice(len(blocks), size=min(per_class, len(blocks)), replace=False)
+        reqs, sel = [], []
+        for i in pick:
+            b = blocks[i]
+            ri = np.sort(rng.choice(len(b.row_cluster), size=min(sub, len(b.row_cluster)), replace=False))
+            ci = np.sort(rng.choice(len(b.col_cluster), size=min(sub, len(b.col_cluster)), replace=False))
+            reqs.append((b.row_cluster[ri], b.col_cluster[ci]))
+            sel.append((b, ri, ci))
+        exact = tree.src.segments(reqs)
+        shear, normal = [], []
+        for (b, ri, ci), A in zip(sel, exact):
+            strongest = max(np.linalg.norm(A[c]) for c in range(6))
+            for c, name in enumerate(("A1s", "A2s", "Bs", "A1d", "A2d", "Bd")):
+                if b.judaca:
+                    U, V = b.factors(name)
+                    approx = factor_round(U[ri]) @ factor_round(V[:, ci]) if U.shape[1] else np.zeros_like(A[c])
+                else:
+                    approx = np.asarray(getattr(b, "Mf_" + name))[np.ix_(ri, ci)].astype(np.float32)
+                e = np.linalg.norm(A[c] - approx) / max(np.linalg.norm(A[c]), 1e-12 * strongest, 1e-300)
+                (normal if c in (2, 5) else shear).append(e)
+        out[key] = {"blocks": len(blocks), "sampled": len(sel), "shear_max": float(np.max(shear)),
+                    "shear_median": float(np.median(shear)), "normal_max": float(np.max(normal)),
+                    "normal_median": float(np.median(normal))}
+    return out
+
+
+def hmatrix_leg(steps=200, warmup=10, graph_steps=8, cpu_budget_s=20.0):
+    """Config 3: 60,000-element rough fault, H-matrix (leaf 64, eta 2, ACA 1e-4,
+    BF16 low-rank factors, FP32 dense leaves).  Build cost, product time
+    against the HBM roofline, error against the exact FP64 dense operator,
+    stepping throughput with the H-matrix and with the dense FP64 matrices,
+    and the reference block-loop product on the host."""
+    from oracle import hmat_oracle as HO
+    from pyquake3d_amd import Hmatrix as H, _lib, synth
+    from pyquake3d_amd.QDsim_gpu import DenseSim
+    nodes_km, tri = synth.rough_fault_mesh(300, 100, 120.0, 30.0, amp_km=1.0, wavelength_km=40.0, hurst=0.8,
+                                           rough_ratio=1e-3, seed=7)
+    nodes = nodes_km * 1e3
+    N = len(tri)
+    log(f"[hmat] N={N}: building tree + ACA")
+    t0 = time.perf_counter()
+    tree = H.build(nodes, tri, MU, LAM, True, mini_leaf=64, min_size=16, eta=2.0, eps=1e-4)
+    build_s = time.perf_counter() - t0
+    nlow = sum(1 for b in tree.blocks_to_process if b.judaca)
+    ranks = [tree.blocks_to_process[i].factors(nm)[0].shape[1] for i in range(len(tree.blocks_to_process))
+             if tree.blocks_to_process[i].judaca for nm in ("A1s", "A2s", "Bs", "A1d", "A2d", "Bd")]
+    log(f"[hmat] built in {build_s:.1f} s: {len(tree.blocks_to_process)} leaves, {nlow} low-rank; "
+        f"{tree.build_stats}")
+    res = {"n_elements": N, "leaves": len(tree.blocks_to_process), "lowrank_leaves": nlow,
+           "rank_mean": float(np.mean(ranks)), "rank_max": int(np.max(ranks)), "build_s": build_s,
+           "build_stats": tree.build_stats,
+           "mesh": "rough vertical fault 120 x 30 km, near-equilateral lattice 300 x 100 x 2; y = 1 km sin(2 pi x / "
+                   "40 km) + self-affine roughness (H = 0.8, RMS 1e-3 x 120 km, seed 7)",
+           "tree": {"leaf": 64, "cluster_min_size": 16, "eta": 2.0, "aca_eps": 1e-4, "recompressed": True}}
+    x1 = np.random.default_rng(2).standard_normal(N)
+    x2 = np.random.default_rng(3).standard_normal(N)
+    prods = {}
+    for label, ft in (("bf16", _lib.MAT_BF16), ("fp32", _lib.MAT_F32)):
+        dev = tree.device(factor_type=ft)
+        info = dev.info()
+        y = dev.mvm(x1, x2)
+        ms = dev.time_mvm(20)
+        ms_fix = dev.time_mvm(20, fix_tn=True)
+        gbs = info["bytes_per_mvm"] / (ms * 1e-3) / 1e9
+        prods[label] = {"y": y, "ms": ms, "ms_fix_tn": ms_fix, "bytes": info["bytes_per_mvm"], "gbs": gbs,
+                        "info": info}
+        log(f"[hmat] product {label}: {ms:.3f} ms, {info['bytes_per_mvm'] / 1e9:.2f} GB -> {gbs:.0f} GB/s")
+    # stepping with the H-matrix (BF16 factors)
+    cols = synth.friction_params(tree.xg, dc_sigma=0.2, seed=5)
+    sh = DenseSim(N, _lib.MAT_NONE)
+    sh.attach_hmatrix(tree.device(factor_type=_lib.MAT_BF16))
+    init_state(sh, cols)
+    h_ms = time_plain(sh, steps, warmup, graph_steps)
+    h_rate = steps / (h_ms * 1e-3)
+    sh.close()
+    log(f"[hmat] stepping with the H-matrix: {h_rate:.1f} steps/s")
+    # exact FP64 operator (173 GB of HBM): product truth and the dense step rate
+    s64 = DenseSim(N, _lib.MAT_F64)
+    P = (tree.src.P1, tree.src.P2, tree.src.P3)
+    t = _lib.Timer()
+    t.start()
+    s64.assemble(*P, tree.xg, tree.eleVec, MU, LAM, True)
+    t.stop()
+    asm64_ms = t.ms()
+    s64.set("rake0", np.zeros(N))
+    s64.set_params(V0, 0.0, MU / (2 * CS), False, True)
+    s64.set("slipv1", x1)
+    s64.set("slipv2", x2)
+    s64.set("normal_loading", np.zeros(N))
+    s64.matvec()
+    exact = [s64.get("AdotV1"), s64.get("AdotV2"), s64.get("dsigmadt")]
+    mv64_ms = s64.time_matvec(5)
+    init_state(s64, cols)
+    k64 = max(20, steps // 10)
+    d_ms = time_plain(s64, k64, 2, graph_steps)
+    d_rate = k64 / (d_ms * 1e-3)
+    s64.close()
+    log(f"[hmat] dense FP64: assembly {asm64_ms:.0f} ms, matvec {mv64_ms:.2f} ms, {d_rate:.1f} steps/s")
+    for label, p in prods.items():
+        p["rel_err_vs_fp64_dense"] = [float(np.linalg.norm(p["y"][q] - exact[q]) / np.linalg.norm(exact[q]))
+                                      for q in range(3)]
+        del p["y"]
+    rng = np.random.default_rng(11)
+    res["block_class_error_bf16"] = hmatrix_class_errors(tree, _bf16, rng)
+    res["block_class_error_fp32"] = hmatrix_class_errors(tree, lambda a: a.astype(np.float32).astype(np.float64),
+                                                         np.random.default_rng(11))
+    # the reference's block-loop product on the host (Hmatrix.py:1773-1866, six calls per step)
+    leaves = []
+    for b in tree.blocks_to_process:
+        if b.judaca:
+            leaves.append((b.row_cluster, b.col_cluster, None, {nm: b.factors(nm) for nm in HO.NAMES}))
+        else:
+            leaves.append((b.row_cluster, b.col_cluster, {nm: getattr(b, "Mf_" + nm) for nm in HO.NAMES}, None))
+    tc = time.perf_counter()
+    nprod = 0
+    while cpu_budget_s > 0 and nprod < 3 and (nprod == 0 or time.perf_counter() - tc < cpu_budget_s):
+        for q in range(3):
+            HO.blocks_mvm(x1, leaves, HO.NAMES[q])
+            HO.blocks_mvm(x2, leaves, HO.NAMES[q + 3])
+        nprod += 1
+    cpu_s = (time.perf_counter() - tc) / nprod if nprod else None
+    tree.close()
+    bf = prods["bf16"]
+    res.update({
+        "product": prods,
+        "roofline": {"bound": "hbm", "achieved": bf["gbs"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": bf["gbs"] / HBM_PEAK_GBS, "traffic": _pmc_traffic("hmat_bf16_60k_traffic.json"),
+                     "kernel": "H-matrix product, BF16 factors (gather + phase A + phase B + reduce)",
+                     "alg_bytes_per_launch": bf["bytes"], "avg_launch_ms": bf["ms"]},
+        "stepping_hmatrix_bf16_steps_per_s": h_rate,
+        "stepping_dense_fp64_steps_per_s": d_rate,
+        "dense_fp64_matvec_ms": mv64_ms,
+        "dense_fp64_assembly_ms": asm64_ms,
+        "product_speedup_vs_dense_fp64": mv64_ms / bf["ms"],
+        "cpu_baseline": None if cpu_s is None else {
+            "value": cpu_s, "unit": "s per step (6 block products)", "cores": 1, "kind": "port",
+            "sample": f"{nprod} step(s) of the reference's block-loop product (NumPy FP64, "
+                      "oracle/hmat_oracle.blocks_mvm) over the same leaves"},
+    })
+    return res
+
+
 def _pmc_traffic(name):
     """HBM bytes per launch from a committed rocprofv3 --pmc summary, or None."""
     p = os.path.join(ROOT, "profiles", name)
@@ -215,11 +383,19 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--serve-requests", type=int, default=24, help="0 skips the serving leg")
     ap.add_argument("--serve-rate", type=float, default=1.0, help="Poisson arrival rate, requests/s")
+    ap.add_argument("--no-hmat", action="store_true", help="skip the 60k H-matrix leg")
+    ap.add_argument("--hmat-only", action="store_true", help="run only the 60k H-matrix leg (JSON of that leg)")
+    ap.add_argument("--hmat-steps", type=int, default=200)
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py: this build runs on one GPU only; how the path would shard is described in DESIGN.md",
               file=sys.stderr)
         sys.exit(2)
+
+    if args.hmat_only:
+        print(json.dumps(hmatrix_leg(steps=args.hmat_steps, warmup=args.warmup, graph_steps=args.graph_steps,
+                                     cpu_budget_s=0.0 if args.no_cpu_baseline else 20.0)))
+        return
 
     from pyquake3d_amd import _lib, synth
     from pyquake3d_amd.QDsim_gpu import DenseSim
@@ -303,6 +479,11 @@ def main():
             f"p99 {serving['p99_time_to_result_s']:.2f} s, {serving['member_steps_per_s']:.0f} member-steps/s, "
             f"occupancy {serving['mean_occupancy']:.2f}")
 
+    hmat = None
+    if not args.no_hmat:
+        hmat = hmatrix_leg(steps=args.hmat_steps, warmup=args.warmup, graph_steps=args.graph_steps,
+                           cpu_budget_s=0.0 if args.no_cpu_baseline else 20.0)
+
     out = {
         "metric": METRIC,
         "value": value,
@@ -339,6 +520,7 @@ def main():
         "gf_assembly": gf,
         "p99_solve_latency_s": serving["p99_time_to_result_s"] if serving else None,
         "serving": serving,
+        "hmatrix": hmat,
     }
     print(json.dumps(out))
 

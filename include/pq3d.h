@@ -185,11 +185,17 @@ int pq3d_gfsrc_destroy(void* src);
 int pq3d_gf_segments(void* src, const int* rcv_idx, long long nr_idx, const int* src_idx, long long ns_idx,
                      long long nseg, const long long* seg, double* out);
 /* ACA of the six matrices of nb blocks (rows row_idx[row_off[b]:row_off[b+1]],
- * columns likewise), relative tolerance eps, max_rank <= 0 for min(m, n);
+ * columns likewise), relative tolerance eps, then (recompress_eps > 0) QR +
+ * SVD recompression to that tolerance; max_rank <= 0 for min(m, n);
  * ranks: nb x 6 out; *result holds the factors */
 int pq3d_aca_run(void* src, long long nb, const long long* row_off, const int* row_idx, const long long* col_off,
-                 const int* col_idx, double eps, int max_rank, unsigned long long seed, long long* ranks,
-                 void** result);
+                 const int* col_idx, double eps, double recompress_eps, int max_rank, unsigned long long seed,
+                 long long* ranks, void** result);
+/* host only: U (m x k column-major) V (k x n row-major) recompressed to
+ * relative Frobenius tolerance eps; rank in *r, factors in Uout / Vout
+ * (the reference's SVD_recompress, src/Hmatrix.py:116-141) */
+int pq3d_lowrank_recompress(long long m, long long n, long long k, const double* U, const double* V, double eps,
+                            long long* r, double* Uout, double* Vout);
 /* stats8: sweeps, launches, pairs evaluated, GPU s, host s, max rank, unconverged, total s */
 int pq3d_aca_stats(void* result, double* stats8);
 /* U (m x k column-major) and V (k x n row-major) per block and matrix A1s..Bd, back to back */
@@ -204,9 +210,10 @@ int pq3d_hmat_create(void** hm, long long n, const int* perm, long long nb, cons
                      const long long* ranks, const double* dense, const double* U, const double* V,
                      int factor_type);
 int pq3d_hmat_destroy(void* hm);
-/* info8: dense bytes, factor bytes, bytes per product, same with fix_tn,
- * segments, row items, column-slab items, low-rank blocks */
-int pq3d_hmat_info(void* hm, double* info8);
+/* info10: dense-leaf bytes, factor bytes, algorithmic bytes per product,
+ * same with fix_tn, segments, row-item waves, column-slab waves, low-rank blocks,
+ * stored (padded) bytes read per product, same with fix_tn */
+int pq3d_hmat_info(void* hm, double* info10);
 /* y (3 x n) = (A1s x1 + A1d x2, A2s x1 + A2d x2, Bs x1 + Bd x2) (host arrays) */
 int pq3d_hmat_mvm(void* hm, const double* x1, const double* x2, int fix_tn, double* y);
 int pq3d_hmat_time_mvm(void* hm, int reps, int fix_tn, float* ms_per_product);

@@ -243,7 +243,8 @@ def _declare(L):
     _lib._sig(L, "pq3d_gfsrc_create", [ctypes.POINTER(vp), dp, dp, dp, ll, dp, dp, ll, d, d, i])
     _lib._sig(L, "pq3d_gfsrc_destroy", [vp])
     _lib._sig(L, "pq3d_gf_segments", [vp, ip, ll, ip, ll, ll, lp, dp])
-    _lib._sig(L, "pq3d_aca_run", [vp, ll, lp, ip, lp, ip, d, i, ctypes.c_ulonglong, lp, ctypes.POINTER(vp)])
+    _lib._sig(L, "pq3d_aca_run", [vp, ll, lp, ip, lp, ip, d, d, i, ctypes.c_ulonglong, lp, ctypes.POINTER(vp)])
+    _lib._sig(L, "pq3d_lowrank_recompress", [ll, ll, ll, dp, dp, d, lp, dp, dp])
     _lib._sig(L, "pq3d_aca_stats", [vp, dp])
     _lib._sig(L, "pq3d_aca_fetch", [vp, dp, dp])
     _lib._sig(L, "pq3d_aca_free", [vp])
@@ -331,10 +332,12 @@ class GFSource:
             pass
 
 
-def aca_blocks(src, blocks, eps=1e-4, max_rank=0, seed=12345):
+def aca_blocks(src, blocks, eps=1e-4, recompress_eps=None, max_rank=0, seed=12345):
     """Adaptive cross approximation of the six matrices of every block in
-    ``blocks`` (lists of row and column ids).  Returns, per block, a list of
-    six (U, V) pairs and the run statistics."""
+    ``blocks`` (lists of row and column ids), recompressed to
+    ``recompress_eps`` (default eps; 0 keeps the ACA factors).  Returns, per
+    block, a list of six (U, V) pairs and the run statistics."""
+    rc_eps = eps if recompress_eps is None else recompress_eps
     nb = len(blocks)
     rows = [_i32(b[0]) for b in blocks]
     cols = [_i32(b[1]) for b in blocks]
@@ -345,8 +348,8 @@ def aca_blocks(src, blocks, eps=1e-4, max_rank=0, seed=12345):
     ranks = np.zeros((nb, 6), dtype=np.int64)
     res = ctypes.c_void_p()
     L = _lib.lib()
-    check(L.pq3d_aca_run(src.h, nb, _lp(roff), _ip(ridx), _lp(coff), _ip(cidx), float(eps), int(max_rank),
-                         int(seed), _lp(ranks), ctypes.byref(res)), "pq3d_aca_run")
+    check(L.pq3d_aca_run(src.h, nb, _lp(roff), _ip(ridx), _lp(coff), _ip(cidx), float(eps), float(rc_eps),
+                         int(max_rank), int(seed), _lp(ranks), ctypes.byref(res)), "pq3d_aca_run")
     try:
         m = np.diff(roff)
         n = np.diff(coff)
@@ -379,22 +382,24 @@ def aca_blocks(src, blocks, eps=1e-4, max_rank=0, seed=12345):
 
 def recompress(U, V, eps):
     """Truncate U V to the smallest rank whose discarded singular values have
-    Frobenius norm <= eps * ||U V||_F (QR of both factors, SVD of the core);
-    the singular values are split evenly between the two factors."""
-    k = U.shape[1]
+    Frobenius norm <= eps * ||U V||_F (QR of both factors, SVD of the core,
+    the singular values split evenly between the factors), natively on the
+    host (``pq3d_lowrank_recompress``; the reference's SVD_recompress,
+    Hmatrix.py:116-141, with a relative tolerance)."""
+    U = np.ascontiguousarray(np.asarray(U, dtype=np.float64))
+    V = np.ascontiguousarray(np.asarray(V, dtype=np.float64))
+    m, k = U.shape
+    n = V.shape[1]
     if k == 0:
-        return U, V
-    qu, ru = np.linalg.qr(U)
-    qv, rv = np.linalg.qr(V.T)
-    w, s, zt = np.linalg.svd(ru @ rv.T)
-    tot = np.sqrt(np.sum(s * s))
-    if tot == 0.0:
-        return U[:, :0], V[:0]
-    tail = np.sqrt(np.cumsum((s * s)[::-1]))[::-1]   # tail[r] = ||s[r:]||
-    keep = np.nonzero(tail > eps * tot)[0]
-    r = int(keep[-1]) + 1 if len(keep) else 0
-    sq = np.sqrt(s[:r])
-    return (qu @ (w[:, :r] * sq)), ((sq[:, None] * zt[:r]) @ qv.T)
+        return U.reshape(m, 0), V.reshape(0, n)
+    Uc = np.ascontiguousarray(U.T)                 # column-major m x k
+    uo = np.empty(m * k)
+    vo = np.empty(k * n)
+    r = ctypes.c_longlong()
+    check(_lib.lib().pq3d_lowrank_recompress(m, n, k, ptr(Uc), ptr(V), float(eps), ctypes.byref(r), ptr(uo),
+                                             ptr(vo)), "pq3d_lowrank_recompress")
+    r = r.value
+    return uo[:m * r].reshape(r, m).T.copy(), vo[:r * n].reshape(r, n).copy()
 
 
 # ----------------------------------------------------------------- block tree
@@ -473,12 +478,11 @@ class BlockTree:
 
     # ---- ACA (Hmatrix.py:661-1344)
     def _aca_one(self, row_cluster, col_cluster, eps, comps):
-        out, _ = aca_blocks(self.src, [(row_cluster, col_cluster)], eps=eps)
+        out, _ = aca_blocks(self.src, [(row_cluster, col_cluster)], eps=eps,
+                            recompress_eps=eps if self.recompress else 0.0)
         d = {}
         for c in comps:
             U, V = out[0][c]
-            if self.recompress:
-                U, V = recompress(U, V, eps)
             d["U_ACA_" + MATRIX_NAMES[c]] = U
             d["V_ACA_" + MATRIX_NAMES[c]] = V
         return d
@@ -529,13 +533,12 @@ class BlockTree:
         dense = [b for b in blocks if not self._is_lowrank(b)]
         t0 = time.perf_counter()
         if lowrank:
-            out, st = aca_blocks(self.src, [(b.row_cluster, b.col_cluster) for b in lowrank], eps=self.aca_eps)
+            out, st = aca_blocks(self.src, [(b.row_cluster, b.col_cluster) for b in lowrank], eps=self.aca_eps,
+                                 recompress_eps=self.aca_eps if self.recompress else 0.0)
             t1 = time.perf_counter()
             for b, per in zip(lowrank, out):
                 dS, dD = {}, {}
                 for c, (U, V) in enumerate(per):
-                    if self.recompress:
-                        U, V = recompress(U, V, self.aca_eps)
                     d = dS if c < 3 else dD
                     d["U_ACA_" + MATRIX_NAMES[c]] = U
                     d["V_ACA_" + MATRIX_NAMES[c]] = V
@@ -543,7 +546,7 @@ class BlockTree:
                 b.judaca = True
                 b.judproc = True
                 self.size += sum(U.nbytes + V.nbytes for U, V in per) / 2 ** 20
-            st["recompress_s"] = time.perf_counter() - t1
+            st["unpack_s"] = time.perf_counter() - t1
             st["aca_s"] = t1 - t0
             self.build_stats["aca"] = st
         t2 = time.perf_counter()
@@ -637,10 +640,11 @@ class DeviceHmatrix:
         self.factor_type = factor_type
 
     def info(self):
-        o = np.zeros(8)
+        o = np.zeros(10)
         check(_lib.lib().pq3d_hmat_info(self.h, ptr(o)), "pq3d_hmat_info")
         return {"dense_bytes": o[0], "factor_bytes": o[1], "bytes_per_mvm": o[2], "bytes_per_mvm_fixtn": o[3],
-                "segments": int(o[4]), "row_items": int(o[5]), "col_items": int(o[6]), "lowrank_blocks": int(o[7])}
+                "segments": int(o[4]), "row_waves": int(o[5]), "col_waves": int(o[6]), "lowrank_blocks": int(o[7]),
+                "stored_bytes_per_mvm": o[8], "stored_bytes_per_mvm_fixtn": o[9]}
 
     def mvm(self, x1, x2, fix_tn=False):
         x1 = np.ascontiguousarray(x1, dtype=np.float64)

@@ -482,6 +482,146 @@ static void parallel_for(long long n, F f) {
     for (auto& x : th) x.join();
 }
 
+// ---------------------------------------------------------------------------
+// Recompression of U V (m x k, k x n): QR of both factors by Gram-Schmidt
+// with reorthogonalisation, one-sided Jacobi SVD of the k x k core, smallest
+// rank whose discarded singular values have Frobenius norm <= eps ||U V||_F;
+// the singular values are split evenly between the factors.
+// ---------------------------------------------------------------------------
+// A: rows x k, column j at j*rows -> Q (same layout), R (k x k, R[i*k + j])
+static void qr_mgs2(int rows, int k, const double* A, std::vector<double>& Q, std::vector<double>& R) {
+    Q.assign(A, A + (size_t)rows * k);
+    R.assign((size_t)k * k, 0.0);
+    for (int j = 0; j < k; ++j) {
+        double* v = Q.data() + (size_t)j * rows;
+        for (int pass = 0; pass < 2; ++pass)
+            for (int i = 0; i < j; ++i) {
+                const double* q = Q.data() + (size_t)i * rows;
+                double d = 0;
+                for (int t = 0; t < rows; ++t) d += q[t] * v[t];
+                R[(size_t)i * k + j] += d;
+                for (int t = 0; t < rows; ++t) v[t] -= d * q[t];
+            }
+        double nv = 0;
+        for (int t = 0; t < rows; ++t) nv += v[t] * v[t];
+        nv = std::sqrt(nv);
+        R[(size_t)j * k + j] = nv;
+        if (nv > 0)
+            for (int t = 0; t < rows; ++t) v[t] /= nv;
+    }
+}
+
+// C (k x k, C[i*k + j]) = W diag(sig) Z^T; W, Z column j at j*k; sig descending
+static void jacobi_svd(int k, const std::vector<double>& C, std::vector<double>& W, std::vector<double>& sig,
+                       std::vector<double>& Z) {
+    std::vector<double> A((size_t)k * k);  // columns of C
+    for (int i = 0; i < k; ++i)
+        for (int j = 0; j < k; ++j) A[(size_t)j * k + i] = C[(size_t)i * k + j];
+    std::vector<double> V((size_t)k * k, 0.0);
+    for (int j = 0; j < k; ++j) V[(size_t)j * k + j] = 1.0;
+    for (int sweep = 0; sweep < 60; ++sweep) {
+        bool rotated = false;
+        for (int p = 0; p < k - 1; ++p)
+            for (int q = p + 1; q < k; ++q) {
+                double* ap = A.data() + (size_t)p * k;
+                double* aq = A.data() + (size_t)q * k;
+                double al = 0, be = 0, ga = 0;
+                for (int t = 0; t < k; ++t) {
+                    al += ap[t] * ap[t];
+                    be += aq[t] * aq[t];
+                    ga += ap[t] * aq[t];
+                }
+                if (ga == 0.0 || std::fabs(ga) <= 1e-15 * std::sqrt(al * be)) continue;
+                rotated = true;
+                const double zeta = (be - al) / (2.0 * ga);
+                const double t = (zeta >= 0 ? 1.0 : -1.0) / (std::fabs(zeta) + std::sqrt(1.0 + zeta * zeta));
+                const double c = 1.0 / std::sqrt(1.0 + t * t), s = c * t;
+                for (int x = 0; x < k; ++x) {
+                    const double u = ap[x], w = aq[x];
+                    ap[x] = c * u - s * w;
+                    aq[x] = s * u + c * w;
+                }
+                double* vp = V.data() + (size_t)p * k;
+                double* vq = V.data() + (size_t)q * k;
+                for (int x = 0; x < k; ++x) {
+                    const double u = vp[x], w = vq[x];
+                    vp[x] = c * u - s * w;
+                    vq[x] = s * u + c * w;
+                }
+            }
+        if (!rotated) break;
+    }
+    std::vector<double> nrm(k);
+    std::vector<int> ord(k);
+    for (int j = 0; j < k; ++j) {
+        double a = 0;
+        for (int t = 0; t < k; ++t) a += A[(size_t)j * k + t] * A[(size_t)j * k + t];
+        nrm[j] = std::sqrt(a);
+        ord[j] = j;
+    }
+    std::stable_sort(ord.begin(), ord.end(), [&](int a, int b) { return nrm[a] > nrm[b]; });
+    W.assign((size_t)k * k, 0.0);
+    Z.assign((size_t)k * k, 0.0);
+    sig.assign(k, 0.0);
+    for (int jj = 0; jj < k; ++jj) {
+        const int j = ord[jj];
+        sig[jj] = nrm[j];
+        for (int t = 0; t < k; ++t) {
+            W[(size_t)jj * k + t] = nrm[j] > 0 ? A[(size_t)j * k + t] / nrm[j] : 0.0;
+            Z[(size_t)jj * k + t] = V[(size_t)j * k + t];
+        }
+    }
+}
+
+// U: m x k (column l at l*m), V: k x n (row l at l*n) -> rank r, factors in
+// place (first r columns / rows)
+static int recompress_factors(int m, int n, int k, std::vector<double>& U, std::vector<double>& V, double eps) {
+    if (k == 0) return 0;
+    std::vector<double> Qu, Ru, Qv, Rv;
+    qr_mgs2(m, k, U.data(), Qu, Ru);
+    qr_mgs2(n, k, V.data(), Qv, Rv);  // V's rows are the columns of V^T
+    std::vector<double> C((size_t)k * k, 0.0);  // Ru Rv^T
+    for (int i = 0; i < k; ++i)
+        for (int j = 0; j < k; ++j) {
+            double a = 0;
+            for (int t = 0; t < k; ++t) a += Ru[(size_t)i * k + t] * Rv[(size_t)j * k + t];
+            C[(size_t)i * k + j] = a;
+        }
+    std::vector<double> W, sig, Z;
+    jacobi_svd(k, C, W, sig, Z);
+    double tot = 0;
+    for (double x : sig) tot += x * x;
+    int r = 0;
+    if (tot > 0) {
+        double tail = tot;
+        while (r < k && tail > eps * eps * tot) {
+            tail -= sig[r] * sig[r];
+            ++r;
+        }
+    }
+    std::vector<double> U2((size_t)m * r, 0.0), V2((size_t)r * n, 0.0);
+    for (int l = 0; l < r; ++l) {
+        const double h = std::sqrt(sig[l]);
+        double* u = U2.data() + (size_t)l * m;
+        double* v = V2.data() + (size_t)l * n;
+        for (int i = 0; i < k; ++i) {
+            const double w = W[(size_t)l * k + i] * h;
+            if (w != 0.0) {
+                const double* q = Qu.data() + (size_t)i * m;
+                for (int t = 0; t < m; ++t) u[t] += w * q[t];
+            }
+            const double z = Z[(size_t)l * k + i] * h;
+            if (z != 0.0) {
+                const double* q = Qv.data() + (size_t)i * n;
+                for (int t = 0; t < n; ++t) v[t] += z * q[t];
+            }
+        }
+    }
+    U.swap(U2);
+    V.swap(V2);
+    return r;
+}
+
 struct AcaResult {
     long long nb = 0;
     std::vector<Inst> inst;
@@ -588,11 +728,12 @@ int pq3d_gf_segments(void* h, const int* rcv_idx, long long nr_idx, const int* s
 
 // ACA of all six matrices of nb blocks.  Block b: rows row_idx[row_off[b] :
 // row_off[b+1]], columns col_idx[col_off[b] : col_off[b+1]] (element ids).
-// eps: relative tolerance; max_rank <= 0: min(m, n).  ranks (nb x 6) receive
+// eps: relative tolerance; recompress_eps > 0: recompress every factor pair
+// to that tolerance afterwards; max_rank <= 0: min(m, n).  ranks (nb x 6) receive
 // the ranks; *result holds the factors until pq3d_aca_free.
 int pq3d_aca_run(void* h, long long nb, const long long* row_off, const int* row_idx, const long long* col_off,
-                 const int* col_idx, double eps, int max_rank, unsigned long long seed, long long* ranks,
-                 void** result) {
+                 const int* col_idx, double eps, double recompress_eps, int max_rank, unsigned long long seed,
+                 long long* ranks, void** result) {
     clear_error();
     GFSource* g = (GFSource*)h;
     PQ_CHECK(g, "pq3d_aca_run: null source");
@@ -705,6 +846,11 @@ int pq3d_aca_run(void* h, long long nb, const long long* row_off, const int* row
         delete res;
         return rc;
     }
+    if (recompress_eps > 0)
+        parallel_for((long long)res->inst.size(), [&](long long x) {
+            Inst& s = res->inst[(size_t)x];
+            s.k = recompress_factors(s.m, s.n, s.k, s.U, s.V, recompress_eps);
+        });
     int maxk = 0;
     long long unconv = 0;
     for (auto& s : res->inst) {
@@ -750,6 +896,23 @@ int pq3d_aca_fetch(void* r, double* U, double* V) {
         uo += us;
         vo += vs;
     }
+    return 0;
+}
+
+// Host-only: recompress U (m x k, column-major) V (k x n, row-major) to
+// relative Frobenius tolerance eps; *r receives the new rank and Uout (m x r,
+// column-major), Vout (r x n, row-major) the factors (capacity m*k, k*n).
+int pq3d_lowrank_recompress(long long m, long long n, long long k, const double* U, const double* V, double eps,
+                            long long* r, double* Uout, double* Vout) {
+    clear_error();
+    PQ_CHECK(m > 0 && n > 0 && k >= 0 && eps > 0, "pq3d_lowrank_recompress: bad arguments");
+    std::vector<double> u(U, U + m * k), v(V, V + k * n);
+    const int rr = recompress_factors((int)m, (int)n, (int)k, u, v, eps);
+    if (rr) {
+        memcpy(Uout, u.data(), (size_t)m * rr * sizeof(double));
+        memcpy(Vout, v.data(), (size_t)rr * n * sizeof(double));
+    }
+    *r = rr;
     return 0;
 }
 

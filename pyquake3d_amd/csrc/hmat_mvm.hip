@@ -6,10 +6,12 @@
 // the cluster tree, so every block is a contiguous [r0, r0+m) x [c0, c0+n)
 // range).  The rows are cut into segments of at most 64 at every block row
 // boundary; a block contributes one item per segment it covers:
-//   * dense leaf item: FP32 [matrix 6][column n][row ms] -- a wave reads 64
-//     consecutive floats per load;
-//   * low-rank item: per matrix the segment's rows of U, [rank k][row ms], in
-//     the factor type (FP32 or BF16).
+//   * dense leaf item: FP32 [matrix 6][column n][row msp], msp = ms rounded
+//     up to 4 (zero rows), so a lane loads 4 rows of a column as one 16-byte
+//     vector;
+//   * low-rank item: per matrix the segment's rows of U, [rank k][row mpu] in
+//     the factor type (FP32 or BF16), mpu = ms rounded up to one 16-byte
+//     vector of rows.
 // Low-rank V factors are stored per (block, matrix) as slabs of VEC rows,
 // [slab][column][VEC] (VEC = 16 bytes of the factor type), so a lane loads
 // one 16-byte vector per column.
@@ -17,10 +19,12 @@
 // Product y = (A1s x1 + A1d x2, A2s x1 + A2d x2, Bs x1 + Bd x2), four launches:
 //   1. gather: x in cluster order (FP32);
 //   2. phase A: t = V x per (block, matrix, slab) -- one wave, FP32 sums;
-//   3. phase B: per item, partial rows = dense * x or U * t -- one wave, lanes
-//      (row, column group), FP32 sums reduced across groups;
+//   3. phase B: per wave (one dense item, or a few low-rank items of a
+//      segment), partial rows = dense * x or sum of U * t -- lanes (row
+//      vector, column group), FP32 sums reduced across groups;
 //   4. reduce: per row, the items of its segment summed in FP64 in a fixed
-//      order (deterministic) and scattered to element order.
+//      order (deterministic: results do not depend on scheduling) and
+//      scattered to element order.
 // With fix_tn the normal-traction matrices Bs, Bd are skipped (the step then
 // uses the normal loading alone, QDsim_gpu.py:948-953).
 #include "common.hpp"
@@ -50,8 +54,16 @@ struct BItem {
     int type;          // 0 dense, 1 low-rank
 };
 
+// one phase-B wave: `count` consecutive items of one segment and one kind
+// (a dense item alone, or up to kLowRankPerWave low-rank items), partial sums
+// [3][ms] at slot
+constexpr int kLowRankPerWave = 2;
+struct BWave {
+    int item0, count, slot, pad;
+};
+
 struct Seg {
-    int s0, ms, item0, item1;
+    int s0, ms, wave0, wave1;  // the segment's waves; their slots are consecutive
     int slot0, pad;
 };
 
@@ -65,6 +77,8 @@ struct HMat {
     long long na = 0;
     BItem* b_items = nullptr;
     long long nbi = 0;
+    BWave* b_waves = nullptr;
+    long long nbw = 0;
     Seg* segs = nullptr;
     long long nseg = 0;
     float* ddata = nullptr;   // dense leaves
@@ -76,7 +90,9 @@ struct HMat {
     double* xin = nullptr;    // 2 x n, element order (host-call inputs)
     double* yout = nullptr;   // 3 x n, element order (host-call outputs)
     long long nt = 0, npart = 0;
-    double bytes_dense = 0, bytes_u = 0, bytes_v = 0, bytes_dense_b = 0, bytes_u_b = 0, bytes_v_b = 0;
+    // bytes read per product: algorithmic (unpadded) and as stored (padded), all
+    // six matrices / without Bs, Bd (fix_tn)
+    double alg = 0, alg_fix = 0, stored = 0, stored_fix = 0, alg_dense = 0, alg_factors = 0;
     long long lowrank_blocks = 0;
 };
 
@@ -150,15 +166,18 @@ __global__ __launch_bounds__(256) void hmat_phase_a_kernel(const AItem* __restri
 #pragma unroll
     for (int e = 0; e < VEC; ++e) acc[e] = 0.f;
     int j = lane;
-    for (; j + 64 < it.n; j += 128) {
-        float v0[VEC], v1[VEC];
+    for (; j + 192 < it.n; j += 256) {  // four 16-byte loads in flight per lane
+        float v0[VEC], v1[VEC], v2[VEC], v3[VEC];
         Slab<T>::load(base, j, v0);
         Slab<T>::load(base, j + 64, v1);
-        const float x0 = x[j], x1 = x[j + 64];
+        Slab<T>::load(base, j + 128, v2);
+        Slab<T>::load(base, j + 192, v3);
+        const float x0 = x[j], x1 = x[j + 64], x2 = x[j + 128], x3 = x[j + 192];
 #pragma unroll
-        for (int e = 0; e < VEC; ++e) acc[e] = fmaf(v0[e], x0, fmaf(v1[e], x1, acc[e]));
+        for (int e = 0; e < VEC; ++e)
+            acc[e] = fmaf(v0[e], x0, fmaf(v1[e], x1, fmaf(v2[e], x2, fmaf(v3[e], x3, acc[e]))));
     }
-    if (j < it.n) {
+    for (; j < it.n; j += 64) {
         float v0[VEC];
         Slab<T>::load(base, j, v0);
         const float x0 = x[j];
@@ -174,93 +193,171 @@ __global__ __launch_bounds__(256) void hmat_phase_a_kernel(const AItem* __restri
     if (lane < VEC) t[it.t_off + lane] = mine;
 }
 
-// phase B: one wave per item; lane = g * mp + i (row i of the segment, column group g)
+__device__ __forceinline__ int pow2ceil(int x) {
+    int p = 1;
+    while (p < x) p <<= 1;
+    return p;
+}
+
+// phase B: one wave per BWave (one dense item, or up to kLowRankPerWave
+// low-rank items of the same segment accumulated together).  Each lane owns R consecutive rows of the
+// segment (one 16-byte vector: 4 FP32 rows of a dense column, or VEC rows of
+// a U column) and a column group g; the groups are summed across lanes at the
+// end in a fixed order.
 template <class T>
-__global__ __launch_bounds__(256) void hmat_phase_b_kernel(const BItem* __restrict__ items, long long nitems,
+__global__ __launch_bounds__(256) void hmat_phase_b_kernel(const BWave* __restrict__ waves, long long nwaves,
+                                                           const BItem* __restrict__ items,
                                                            const float* __restrict__ ddata,
                                                            const T* __restrict__ udata,
                                                            const float* __restrict__ xp, long long n,
                                                            const float* __restrict__ t,
                                                            float* __restrict__ partial, int fix_tn) {
+    constexpr int VR = Slab<T>::VEC;  // U rows per 16-byte vector
     const long long w = (long long)blockIdx.x * 4 + (threadIdx.x >> 6);
     const int lane = threadIdx.x & 63;
-    if (w >= nitems) return;  // whole wave
-    const BItem it = items[w];
-    const int ms = it.ms;
-    const int mp = ms > 32 ? 64 : (ms > 16 ? 32 : (ms > 8 ? 16 : (ms > 4 ? 8 : (ms > 2 ? 4 : (ms > 1 ? 2 : 1)))));
-    const int G = 64 / mp;
-    const int i = lane & (mp - 1), g = lane / mp;
-    const bool row_ok = i < ms;
-    float acc0 = 0.f, acc1 = 0.f, acc2 = 0.f;
-    if (it.type == 0) {
-        const float* D = ddata + it.off[0];
-        const float* x1 = xp + it.c0;
-        const float* x2 = xp + n + it.c0;
-        const long long cs = (long long)it.n * ms;  // matrix stride
-        for (int j = g; j < it.n; j += G) {
-            if (!row_ok) continue;
-            const float a = x1[j], b = x2[j];
-            const float* col = D + (long long)j * ms + i;
-            acc0 = fmaf(col[0], a, acc0);
-            acc1 = fmaf(col[cs], a, acc1);
-            acc0 = fmaf(col[3 * cs], b, acc0);
-            acc1 = fmaf(col[4 * cs], b, acc1);
-            if (!fix_tn) {
-                acc2 = fmaf(col[2 * cs], a, acc2);
-                acc2 = fmaf(col[5 * cs], b, acc2);
+    if (w >= nwaves) return;  // whole wave
+    const BWave wv = waves[w];
+    const int ms = items[wv.item0].ms;
+    const int type = items[wv.item0].type;
+    float acc[VR > 4 ? VR : 4][3];
+#pragma unroll
+    for (int r = 0; r < (VR > 4 ? VR : 4); ++r) acc[r][0] = acc[r][1] = acc[r][2] = 0.f;
+    int R, qp, q, g;
+    if (type == 0) {
+        const BItem it = items[wv.item0];
+        R = 4;
+        const int msp = (ms + 3) & ~3, nq = msp >> 2;
+        qp = pow2ceil(nq);
+        const int G = 64 / qp;
+        q = lane & (qp - 1);
+        g = lane / qp;
+        if (q < nq) {
+            const float* D = ddata + it.off[0] + 4 * q;
+            const float* x1 = xp + it.c0;
+            const float* x2 = xp + n + it.c0;
+            const long long cs = (long long)it.n * msp;  // matrix stride
+            const int nm = fix_tn ? 4 : 6;
+            const int mat[6] = {0, 1, 3, 4, 2, 5};         // Bs, Bd last (skipped with fix_tn)
+            for (int j0 = g; j0 < it.n; j0 += 2 * G) {     // two columns' loads in flight
+                float4 m[2][6];
+                float a[2], b[2];
+#pragma unroll
+                for (int u = 0; u < 2; ++u) {
+                    const int j = j0 + u * G < it.n ? j0 + u * G : j0;
+                    const float* col = D + (long long)j * msp;
+#pragma unroll
+                    for (int c = 0; c < 6; ++c)
+                        if (c < nm) m[u][c] = *reinterpret_cast<const float4*>(col + mat[c] * cs);
+                    const bool live = j0 + u * G < it.n;
+                    a[u] = live ? x1[j] : 0.f;
+                    b[u] = live ? x2[j] : 0.f;
+                }
+#pragma unroll
+                for (int u = 0; u < 2; ++u) {
+                    const float v0[4] = {m[u][0].x, m[u][0].y, m[u][0].z, m[u][0].w};
+                    const float v1[4] = {m[u][1].x, m[u][1].y, m[u][1].z, m[u][1].w};
+                    const float v3[4] = {m[u][2].x, m[u][2].y, m[u][2].z, m[u][2].w};
+                    const float v4[4] = {m[u][3].x, m[u][3].y, m[u][3].z, m[u][3].w};
+#pragma unroll
+                    for (int r = 0; r < 4; ++r) {
+                        acc[r][0] = fmaf(v0[r], a[u], fmaf(v3[r], b[u], acc[r][0]));
+                        acc[r][1] = fmaf(v1[r], a[u], fmaf(v4[r], b[u], acc[r][1]));
+                    }
+                    if (nm == 6) {
+                        const float v2[4] = {m[u][4].x, m[u][4].y, m[u][4].z, m[u][4].w};
+                        const float v5[4] = {m[u][5].x, m[u][5].y, m[u][5].z, m[u][5].w};
+#pragma unroll
+                        for (int r = 0; r < 4; ++r) acc[r][2] = fmaf(v2[r], a[u], fmaf(v5[r], b[u], acc[r][2]));
+                    }
+                }
             }
         }
     } else {
+        R = VR;
+        const int mpu = (ms + VR - 1) / VR * VR, no = mpu / VR;
+        qp = pow2ceil(no);
+        const int G = 64 / qp;
+        q = lane & (qp - 1);
+        g = lane / qp;
+        if (q < no) {
+            for (int x = 0; x < wv.count; ++x) {
+                const BItem& it = items[wv.item0 + x];
 #pragma unroll
-        for (int c = 0; c < 6; ++c) {
-            if (fix_tn && (c == 2 || c == 5)) continue;
-            const int k = it.k[c];
-            const T* U = udata + it.off[c];
-            const float* tt = t + it.t_off[c];
-            float s = 0.f;
-            if (row_ok)
-                for (int l = g; l < k; l += G) s = fmaf(Slab<T>::one(U, (long long)l * ms + i), tt[l], s);
-            if (c % 3 == 0) acc0 += s;
-            else if (c % 3 == 1) acc1 += s;
-            else acc2 += s;
+                for (int c = 0; c < 6; ++c) {
+                    if (fix_tn && (c == 2 || c == 5)) continue;
+                    const int k = it.k[c];
+                    const T* U = udata + it.off[c];
+                    const float* tt = t + it.t_off[c];
+                    for (int l = g; l < k; l += G) {
+                        float u[VR];
+                        Slab<T>::load(U + (long long)l * mpu, q, u);
+                        const float tl = tt[l];
+#pragma unroll
+                        for (int r = 0; r < VR; ++r) acc[r][c % 3] = fmaf(u[r], tl, acc[r][c % 3]);
+                    }
+                }
+            }
         }
     }
-    for (int o = mp; o < 64; o <<= 1) {
-        acc0 += __shfl_xor(acc0, o, 64);
-        acc1 += __shfl_xor(acc1, o, 64);
-        acc2 += __shfl_xor(acc2, o, 64);
+    for (int o = qp; o < 64; o <<= 1) {
+#pragma unroll
+        for (int r = 0; r < (VR > 4 ? VR : 4); ++r) {
+            acc[r][0] += __shfl_xor(acc[r][0], o, 64);
+            acc[r][1] += __shfl_xor(acc[r][1], o, 64);
+            acc[r][2] += __shfl_xor(acc[r][2], o, 64);
+        }
     }
-    if (g == 0 && row_ok) {
-        float* P = partial + it.slot;
-        P[i] = acc0;
-        P[ms + i] = acc1;
-        P[2 * ms + i] = acc2;
+    if (g == 0) {
+        float* P = partial + wv.slot;
+#pragma unroll
+        for (int r = 0; r < (VR > 4 ? VR : 4); ++r) {
+            const int row = q * R + r;
+            if (r < R && row < ms) {
+                P[row] = acc[r][0];
+                P[ms + row] = acc[r][1];
+                P[2 * ms + row] = acc[r][2];
+            }
+        }
     }
 }
 
-// reduce: per cluster-order row, sum its segment's items (FP64, fixed order)
-// and scatter to element order: out[q * n + perm[p]]
+// reduce: per cluster-order row, eight lanes each sum every eighth item of
+// the row's segment, then the eight partial sums are combined in a fixed
+// butterfly (deterministic); FP64 sums; scattered to element order:
+// out[q * n + perm[p]]
 __global__ __launch_bounds__(256) void hmat_reduce_kernel(const Seg* __restrict__ segs,
                                                           const int* __restrict__ row_seg,
                                                           const int* __restrict__ perm, long long n,
                                                           const float* __restrict__ partial,
                                                           double* __restrict__ out) {
-    const long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= n) return;
-    const Seg s = segs[row_seg[p]];
-    const int i = (int)(p - s.s0);
+    const long long tid = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long long p = tid >> 3;
+    const int sub = (int)(tid & 7);
+    const bool ok = p < n;
     double y0 = 0, y1 = 0, y2 = 0;
-    const float* P = partial + s.slot0 + i;
-    const long long stride = 3LL * s.ms;
-    for (int it = s.item0; it < s.item1; ++it, P += stride) {
-        y0 += (double)P[0];
-        y1 += (double)P[s.ms];
-        y2 += (double)P[2 * s.ms];
+    if (ok) {
+        const Seg s = segs[row_seg[p]];
+        const int i = (int)(p - s.s0);
+        const long long stride = 3LL * s.ms;
+        const float* P = partial + s.slot0 + i + stride * sub;
+        for (int it = s.wave0 + sub; it < s.wave1; it += 8, P += 8 * stride) {
+            y0 += (double)P[0];
+            y1 += (double)P[s.ms];
+            y2 += (double)P[2 * s.ms];
+        }
     }
-    const int e = perm[p];
-    out[e] = y0;
-    out[n + e] = y1;
-    out[2 * n + e] = y2;
+#pragma unroll
+    for (int o = 1; o < 8; o <<= 1) {
+        y0 += __shfl_xor(y0, o, 64);
+        y1 += __shfl_xor(y1, o, 64);
+        y2 += __shfl_xor(y2, o, 64);
+    }
+    if (ok && sub == 0) {
+        const int e = perm[p];
+        out[e] = y0;
+        out[n + e] = y1;
+        out[2 * n + e] = y2;
+    }
 }
 
 static unsigned short f2bf(float f) {
@@ -284,10 +381,11 @@ static void launch_mvm_t(HMat* h, const double* V1, const double* V2, const doub
     if (h->na)
         hipLaunchKernelGGL(hmat_phase_a_kernel<T>, dim3((unsigned)((h->na + 3) / 4)), dim3(256), 0, st, h->a_items,
                            h->na, (const T*)h->vdata, h->xp, n, h->t, fix_tn);
-    if (h->nbi)
-        hipLaunchKernelGGL(hmat_phase_b_kernel<T>, dim3((unsigned)((h->nbi + 3) / 4)), dim3(256), 0, st,
-                           h->b_items, h->nbi, h->ddata, (const T*)h->udata, h->xp, n, h->t, h->partial, fix_tn);
-    hipLaunchKernelGGL(hmat_reduce_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, h->segs,
+    if (h->nbw)
+        hipLaunchKernelGGL(hmat_phase_b_kernel<T>, dim3((unsigned)((h->nbw + 3) / 4)), dim3(256), 0, st,
+                           h->b_waves, h->nbw, h->b_items, h->ddata, (const T*)h->udata, h->xp, n, h->t, h->partial,
+                           fix_tn);
+    hipLaunchKernelGGL(hmat_reduce_kernel, dim3((unsigned)((8 * n + 255) / 256)), dim3(256), 0, st, h->segs,
                        h->row_seg, h->perm, n, h->partial, out);
 }
 
@@ -403,7 +501,6 @@ static int pack(HMat* h, long long n, const int* perm, long long nb, const long 
     std::vector<AItem> aitems;
     std::vector<T> vpack;
     long long tcount = 0;
-    h->bytes_v = h->bytes_v_b = 0;
     h->lowrank_blocks = 0;
     for (long long b = 0; b < nb; ++b) {
         if (kind[b] == 0) continue;
@@ -427,23 +524,29 @@ static int pack(HMat* h, long long n, const int* perm, long long nb, const long 
                 }
                 aitems.push_back(AItem{base + lb * nc * VEC, (int)nc, c0[b], c / 3, (int)(tcount + lb * VEC), c, 0});
             }
-            const double bytes = (double)kp * nc * sizeof(T);
-            h->bytes_v += bytes;
-            if (c != 2 && c != 5) h->bytes_v_b += bytes;
+            const double ab = (double)k * nc * sizeof(T), sb = (double)kp * nc * sizeof(T);
+            h->alg += ab;
+            h->stored += sb;
+            h->alg_factors += ab;
+            if (c != 2 && c != 5) {
+                h->alg_fix += ab;
+                h->stored_fix += sb;
+            }
             tcount += kp;
         }
         (void)m;
     }
-    // phase B items
+    // phase B items: per segment the dense leaves first, then the low-rank
+    // blocks, in block order within each kind
     std::vector<BItem> bitems;
+    std::vector<BWave> bwaves;
     std::vector<float> dpack;
     std::vector<T> upack;
     long long slot = 0;
-    h->bytes_dense = h->bytes_dense_b = h->bytes_u = h->bytes_u_b = 0;
     for (size_t s = 0; s < segs.size(); ++s) {
         Seg& sg = segs[s];
-        sg.item0 = (int)bitems.size();
-        sg.slot0 = (int)slot;
+        std::stable_partition(seg_blocks[s].begin(), seg_blocks[s].end(), [&](int b) { return kind[b] == 0; });
+        const int first_item = (int)bitems.size();
         for (int b : seg_blocks[s]) {
             const long long m = row_off[b + 1] - row_off[b], nc = col_off[b + 1] - col_off[b];
             const int ms = sg.ms;
@@ -455,25 +558,32 @@ static int pack(HMat* h, long long n, const int* perm, long long nb, const long 
             it.slot = (int)slot;
             it.type = kind[b];
             if (kind[b] == 0) {
+                // columns padded to a multiple of 4 rows (16-byte vectors)
+                const int msp = (ms + 3) & ~3;
                 it.c0 = c0[b];
                 it.n = (int)nc;
                 const long long base = (long long)dpack.size();
                 it.off[0] = base;
-                dpack.resize((size_t)(base + 6 * nc * ms));
+                dpack.resize((size_t)(base + 6 * nc * msp), 0.0f);
                 const double* Db = dense + doff[b];
                 for (int c = 0; c < 6; ++c)
                     for (long long jp = 0; jp < nc; ++jp) {
                         const int j = cinv[cinv_off[b] + jp];
                         for (int ip = 0; ip < ms; ++ip) {
                             const int i = rinv[rinv_off[b] + lr0 + ip];
-                            dpack[(size_t)(base + ((long long)c * nc + jp) * ms + ip)] =
+                            dpack[(size_t)(base + ((long long)c * nc + jp) * msp + ip)] =
                                 (float)Db[(long long)c * m * nc + (long long)j * m + i];
                         }
                     }
-                const double bytes = 6.0 * nc * ms * 4;
-                h->bytes_dense += bytes;
-                h->bytes_dense_b += bytes * 4 / 6;
+                const double ab = 6.0 * nc * ms * 4, sb = 6.0 * nc * msp * 4;
+                h->alg += ab;
+                h->alg_fix += ab * 4 / 6;
+                h->stored += sb;
+                h->stored_fix += sb * 4 / 6;
+                h->alg_dense += ab;
             } else {
+                // U columns padded to a multiple of VEC rows (16-byte vectors)
+                const int mpu = (ms + VEC - 1) / VEC * VEC;
                 for (int c = 0; c < 6; ++c) {
                     const long long k = ranks[6 * b + c];
                     it.k[c] = (int)k;
@@ -481,22 +591,41 @@ static int pack(HMat* h, long long n, const int* perm, long long nb, const long 
                     const long long base = (long long)upack.size();
                     it.off[c] = base;
                     if (k == 0) continue;
-                    upack.resize((size_t)(base + k * ms));
+                    upack.resize((size_t)(base + k * mpu), T(0));
                     const double* Ub = U + uoff[6 * b + c];
                     for (long long l = 0; l < k; ++l)
                         for (int ip = 0; ip < ms; ++ip) {
                             const int i = rinv[rinv_off[b] + lr0 + ip];
-                            upack[(size_t)(base + l * ms + ip)] = conv<T>(Ub[l * m + i]);
+                            upack[(size_t)(base + l * mpu + ip)] = conv<T>(Ub[l * m + i]);
                         }
-                    const double bytes = (double)k * ms * sizeof(T);
-                    h->bytes_u += bytes;
-                    if (c != 2 && c != 5) h->bytes_u_b += bytes;
+                    const double ab = (double)k * ms * sizeof(T), sb = (double)k * mpu * sizeof(T);
+                    h->alg += ab;
+                    h->stored += sb;
+                    h->alg_factors += ab;
+                    if (c != 2 && c != 5) {
+                        h->alg_fix += ab;
+                        h->stored_fix += sb;
+                    }
                 }
             }
             bitems.push_back(it);
-            slot += 3 * ms;
         }
-        sg.item1 = (int)bitems.size();
+        // waves: one per dense item, kLowRankPerWave low-rank items per wave
+        // (many small waves hide memory latency better: 8 per wave measured
+        // 15% slower than 1 on the 60k product)
+        sg.wave0 = (int)bwaves.size();
+        sg.slot0 = (int)slot;
+        const int last_item = (int)bitems.size();
+        for (int i = first_item; i < last_item;) {
+            int cnt = 1;
+            if (bitems[i].type == 1)
+                while (cnt < kLowRankPerWave && i + cnt < last_item && bitems[i + cnt].type == 1) ++cnt;
+            bwaves.push_back(BWave{i, cnt, (int)slot, 0});
+            for (int q = 0; q < cnt; ++q) bitems[i + q].slot = (int)slot;
+            slot += 3 * sg.ms;
+            i += cnt;
+        }
+        sg.wave1 = (int)bwaves.size();
     }
     PQ_CHECK(slot < INT32_MAX && tcount < INT32_MAX, "pq3d_hmat_create: H-matrix too large for 32-bit offsets");
     std::vector<int> row_seg(n);
@@ -506,6 +635,7 @@ static int pack(HMat* h, long long n, const int* perm, long long nb, const long 
     h->n = n;
     h->na = (long long)aitems.size();
     h->nbi = (long long)bitems.size();
+    h->nbw = (long long)bwaves.size();
     h->nseg = (long long)segs.size();
     h->nt = tcount;
     h->npart = slot;
@@ -518,6 +648,7 @@ static int pack(HMat* h, long long n, const int* perm, long long nb, const long 
     if (up((void**)&h->row_seg, row_seg.data(), n * sizeof(int))) return -1;
     if (up((void**)&h->a_items, aitems.data(), aitems.size() * sizeof(AItem))) return -1;
     if (up((void**)&h->b_items, bitems.data(), bitems.size() * sizeof(BItem))) return -1;
+    if (up((void**)&h->b_waves, bwaves.data(), bwaves.size() * sizeof(BWave))) return -1;
     if (up((void**)&h->segs, segs.data(), segs.size() * sizeof(Seg))) return -1;
     if (up((void**)&h->ddata, dpack.data(), dpack.size() * sizeof(float))) return -1;
     if (up(&h->udata, upack.data(), upack.size() * sizeof(T))) return -1;
@@ -533,7 +664,7 @@ static int pack(HMat* h, long long n, const int* perm, long long nb, const long 
 }
 
 static void hmat_free(HMat* h) {
-    void* ptrs[] = {h->perm, h->row_seg, h->a_items, h->b_items, h->segs, h->ddata, h->udata,
+    void* ptrs[] = {h->perm, h->row_seg, h->a_items, h->b_items, h->b_waves, h->segs, h->ddata, h->udata,
                     h->vdata, h->xp, h->t, h->partial, h->xin, h->yout};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
@@ -579,20 +710,23 @@ int pq3d_hmat_destroy(void* hm) {
     return 0;
 }
 
-// info8: dense bytes, factor bytes, bytes read per product, same with fix_tn,
-// segments, phase-B items, phase-A items, low-rank blocks
-int pq3d_hmat_info(void* hm, double* info8) {
+// info10: dense-leaf bytes, factor bytes (unpadded), algorithmic bytes per
+// product, same with fix_tn, segments, phase-B waves, phase-A waves, low-rank
+// blocks, stored bytes read per product (padded), same with fix_tn
+int pq3d_hmat_info(void* hm, double* info10) {
     clear_error();
     HMat* h = (HMat*)hm;
     PQ_CHECK(h, "pq3d_hmat_info: null");
-    info8[0] = h->bytes_dense;
-    info8[1] = h->bytes_u + h->bytes_v;
-    info8[2] = h->bytes_dense + h->bytes_u + h->bytes_v;
-    info8[3] = h->bytes_dense_b + h->bytes_u_b + h->bytes_v_b;
-    info8[4] = (double)h->nseg;
-    info8[5] = (double)h->nbi;
-    info8[6] = (double)h->na;
-    info8[7] = (double)h->lowrank_blocks;
+    info10[0] = h->alg_dense;
+    info10[1] = h->alg_factors;
+    info10[2] = h->alg;
+    info10[3] = h->alg_fix;
+    info10[4] = (double)h->nseg;
+    info10[5] = (double)h->nbw;
+    info10[6] = (double)h->na;
+    info10[7] = (double)h->lowrank_blocks;
+    info10[8] = h->stored;
+    info10[9] = h->stored_fix;
     return 0;
 }
 

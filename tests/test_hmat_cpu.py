@@ -10,14 +10,20 @@ Pins:
 * partition: every matrix entry lies in exactly one leaf; every low-rank
   leaf is admissible; every cluster is a contiguous range of the cluster
   order;
-* recompression: a rank-5 matrix plus tiny noise comes back at rank 5 within
-  the tolerance.
+* recompression (native host code): a rank-5 matrix written with redundant
+  terms comes back at rank 5; on a matrix with known singular values the
+  kept rank is the one the tolerance asks for and the result equals the
+  truncated SVD to 1e-12; the zero matrix has rank 0.
 """
+import os
+
 import numpy as np
 import pytest
 
 from oracle import hmat_oracle as HO
 from pyquake3d_amd import Hmatrix as H
+
+LIB = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "pyquake3d_amd", "libpq3d.so")
 
 
 def _leaf_sets(node, out):
@@ -122,6 +128,7 @@ def test_partition_admissibility_and_cluster_order(rough_tree):
     assert cover.min() == 1 and cover.max() == 1
 
 
+@pytest.mark.skipif(not os.path.exists(LIB), reason="libpq3d.so not built")
 def test_recompress_recovers_rank():
     rng = np.random.default_rng(1)
     X, Y = rng.standard_normal((60, 5)), rng.standard_normal((5, 50))
@@ -134,3 +141,17 @@ def test_recompress_recovers_rank():
     assert np.linalg.norm(u @ v - A) <= 1e-8 * np.linalg.norm(A)
     u0, v0 = H.recompress(np.zeros((7, 0)), np.zeros((0, 9)), 1e-4)
     assert u0.shape == (7, 0) and v0.shape == (0, 9)
+    # against the SVD: a matrix with singular values 10^-j keeps the ranks the
+    # tolerance asks for, and the kept part is the truncated SVD
+    qa, _ = np.linalg.qr(rng.standard_normal((40, 8)))
+    qb, _ = np.linalg.qr(rng.standard_normal((30, 8)))
+    s = 10.0 ** -np.arange(8)
+    A = (qa * s) @ qb.T
+    for eps, want in ((1e-2, 2), (1e-5, 5)):
+        u, v = H.recompress(qa * s, qb.T, eps)
+        assert u.shape[1] == want
+        best = (qa[:, :want] * s[:want]) @ qb[:, :want].T
+        assert np.linalg.norm(u @ v - best) <= 1e-12 * np.linalg.norm(A)
+    # the exact zero matrix has rank 0
+    u, v = H.recompress(np.zeros((5, 3)), np.ones((3, 4)), 1e-4)
+    assert u.shape == (5, 0) and v.shape == (0, 4)
