@@ -117,8 +117,10 @@ __global__ __launch_bounds__(64) void ens_gemm_kernel(const float* __restrict__ 
     if (k1 > n) k1 = n;
     const int lr = lane & 15;   // row / member group within a 16-lane block
     const int lk = lane >> 4;   // k within the 4-deep MFMA step
-    const long long row = r0 + 4 * lr;
-    const bool row_ok = row < ld;  // ld is a multiple of 4: the float4 stays inside the column
+    // Loads are branch-free: rows past ld read the last row vector (their
+    // results land in padding rows that nothing reads) and k past the chunk
+    // reads its last column with u = 0, so it adds nothing.
+    const long long row = r0 + 4 * lr < ld ? r0 + 4 * lr : ld - 4;
     const float* mats[6] = {A1s, A2s, Bs, A1d, A2d, Bd};
     v4f acc[3][4][4];
 #pragma unroll
@@ -128,31 +130,35 @@ __global__ __launch_bounds__(64) void ens_gemm_kernel(const float* __restrict__ 
 #pragma unroll
             for (int j = 0; j < 4; ++j) acc[q][i][j] = v4f{0.f, 0.f, 0.f, 0.f};
 
-    for (long long k = k0; k < k1; k += 4) {
+    float4 a[6], u1, u2;
+    bool ok = false;  // the loaded k is inside the chunk (else u counts as 0)
+    auto load = [&](long long k) {
         const long long s = k + lk;
-        const bool s_ok = s < k1;
-        float4 a[6];
+        ok = s < k1;
+        const long long sc = ok ? s : k1 - 1;
 #pragma unroll
-        for (int m = 0; m < 6; ++m) {
-            a[m] = (s_ok && row_ok) ? *reinterpret_cast<const float4*>(mats[m] + (size_t)s * ld + row)
-                                    : make_float4(0.f, 0.f, 0.f, 0.f);
-        }
-        const float4 u1 = s_ok ? *reinterpret_cast<const float4*>(U1 + (size_t)s * 64 + 4 * lr)
-                               : make_float4(0.f, 0.f, 0.f, 0.f);
-        const float4 u2 = s_ok ? *reinterpret_cast<const float4*>(U2 + (size_t)s * 64 + 4 * lr)
-                               : make_float4(0.f, 0.f, 0.f, 0.f);
-        const float uu1[4] = {u1.x, u1.y, u1.z, u1.w};
-        const float uu2[4] = {u2.x, u2.y, u2.z, u2.w};
+        for (int m = 0; m < 6; ++m) a[m] = *reinterpret_cast<const float4*>(mats[m] + (size_t)sc * ld + row);
+        u1 = *reinterpret_cast<const float4*>(U1 + (size_t)sc * 64 + 4 * lr);
+        u2 = *reinterpret_cast<const float4*>(U2 + (size_t)sc * 64 + 4 * lr);
+    };
+    load(k0);
+    for (long long k = k0; k < k1; k += 4) {
+        // this step's operands; the next step's loads are issued before its MFMAs
+        const float as_[3][4] = {{a[0].x, a[0].y, a[0].z, a[0].w}, {a[1].x, a[1].y, a[1].z, a[1].w},
+                                 {a[2].x, a[2].y, a[2].z, a[2].w}};
+        const float ad_[3][4] = {{a[3].x, a[3].y, a[3].z, a[3].w}, {a[4].x, a[4].y, a[4].z, a[4].w},
+                                 {a[5].x, a[5].y, a[5].z, a[5].w}};
+        const float uu1[4] = {ok ? u1.x : 0.f, ok ? u1.y : 0.f, ok ? u1.z : 0.f, ok ? u1.w : 0.f};
+        const float uu2[4] = {ok ? u2.x : 0.f, ok ? u2.y : 0.f, ok ? u2.z : 0.f, ok ? u2.w : 0.f};
+        if (k + 4 < k1) load(k + 4);
 #pragma unroll
         for (int q = 0; q < 3; ++q) {
-            const float as[4] = {a[q].x, a[q].y, a[q].z, a[q].w};
-            const float ad[4] = {a[q + 3].x, a[q + 3].y, a[q + 3].z, a[q + 3].w};
 #pragma unroll
             for (int i = 0; i < 4; ++i)
 #pragma unroll
                 for (int j = 0; j < 4; ++j) {
-                    acc[q][i][j] = __builtin_amdgcn_mfma_f32_16x16x4f32(as[i], uu1[j], acc[q][i][j], 0, 0, 0);
-                    acc[q][i][j] = __builtin_amdgcn_mfma_f32_16x16x4f32(ad[i], uu2[j], acc[q][i][j], 0, 0, 0);
+                    acc[q][i][j] = __builtin_amdgcn_mfma_f32_16x16x4f32(as_[q][i], uu1[j], acc[q][i][j], 0, 0, 0);
+                    acc[q][i][j] = __builtin_amdgcn_mfma_f32_16x16x4f32(ad_[q][i], uu2[j], acc[q][i][j], 0, 0, 0);
                 }
         }
     }
