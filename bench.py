@@ -386,11 +386,24 @@ def main():
     ap.add_argument("--no-hmat", action="store_true", help="skip the 60k H-matrix leg")
     ap.add_argument("--hmat-only", action="store_true", help="run only the 60k H-matrix leg (JSON of that leg)")
     ap.add_argument("--hmat-steps", type=int, default=200)
+    ap.add_argument("--copy-probe-gib", type=float, default=0.0,
+                    help="only copy a device buffer of this size 5 times (calibrates PMC byte counters)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py: this build runs on one GPU only; how the path would shard is described in DESIGN.md",
               file=sys.stderr)
         sys.exit(2)
+
+    if args.copy_probe_gib > 0:
+        from pyquake3d_amd import _lib
+        nbytes = int(args.copy_probe_gib * 2 ** 30) // 256 * 256
+        src, dst = _lib.DeviceBuffer(nbytes), _lib.DeviceBuffer(nbytes)
+        _lib.check(_lib.lib().pq3d_memset(src.ptr, 1, nbytes), "memset")
+        for _ in range(5):
+            _lib.check(_lib.lib().pq3d_memcpy_d2d(dst.ptr, src.ptr, nbytes), "copy")
+        _lib.lib().pq3d_synchronize()
+        print(json.dumps({"copy_bytes": nbytes, "copies": 5}))
+        return
 
     if args.hmat_only:
         print(json.dumps(hmatrix_leg(steps=args.hmat_steps, warmup=args.warmup, graph_steps=args.graph_steps,
