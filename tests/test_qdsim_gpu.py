@@ -11,6 +11,8 @@ borderline accept/reject decision.  Bars:
 * state vectors: relative to their max magnitude, 1e-9 over those steps;
 * traction matvec: FP64 storage 1e-12 of sum|a_ij u_j| per row, FP32 storage
   1e-6, BF16 storage 1e-2;
+* the reference's own 2,000-step CPU run (golden fixture): steps 1-100 within
+  1e-9, the same seismic events (onset 1e-3, peak slip rate 1%);
 * failure handling: a NaN forced into the state from the host gives the
   reference's 'dt is too small' outcome with the state left unchanged.
 """
@@ -200,3 +202,53 @@ def test_dropin_qdsim_api(tmp_path):
     # a second construction loads the cached Green's functions
     sim1 = QDsim_gpu.QDsim(elelst, nodelst, par)
     assert np.array_equal(sim1.A1s, sim0.A1s)
+
+
+def test_device_follows_reference_trajectory():
+    """2,000 device steps (FP64 storage, the reference CPU path's error
+    estimate) against the reference's own simu_forward run
+    (tests/golden/qdsim_trajectory.npz) on bit-identical Green's functions:
+    steps 1-100 within 1e-9 (the device matvec sums in another order than
+    BLAS, so rounding differences start at 1e-16 and grow chaotically); the
+    same seismic events, onset within 1e-3 (measured 1.8e-4: nine days in
+    132 years), peak slip rate within 1%."""
+    import sys
+    from oracle import refbind
+    from pyquake3d_amd import _lib
+    from pyquake3d_amd.QDsim_gpu import DenseSim
+    if not refbind.available():
+        pytest.skip("reference C++ (oracle/_ref) not built")
+    golden = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+    sys.path.insert(0, golden)
+    import make_qdsim_trajectory as G
+    g = dict(np.load(os.path.join(golden, "qdsim_trajectory.npz"), allow_pickle=False))
+    nodes_km, nodes_m, tri, cols = G.case_inputs()
+    mats = G.traction_matrices(nodes_m, tri, workers=8, context="spawn")
+    assert G.matrices_sha256(mats) == str(g["gf_sha256"])
+    n = len(tri)
+    orc = Q.from_init_columns([np.zeros((1, 1))] * 6, cols, V0, VPL, MU, CS, False)
+    s = DenseSim(n, _lib.MAT_F64)
+    for k in range(6):
+        s.upload_matrix(k, mats[k])
+    for name in ("Tno", "Tt1o", "Tt2o", "state", "slipv1", "slipv2", "slipv", "a", "b", "dc", "f0",
+                 "shear_loading", "normal_loading"):
+        s.set(name, getattr(orc, name))
+    s.set("rake0", orc.rake0)
+    s.set_params(V0, VPL, MU / (2 * CS), False, False)
+    ref = g["records"]
+    h, dtn, _ = s.step(1e-3)
+    st = s.state()
+    first = np.array([[st["time"], h, dtn, st["maxV"]]])
+    status, rec = s.run(len(ref) - 1, records=True)
+    assert status == 0
+    rec = np.vstack([first, rec[:, [2, 0, 1, 3]]])
+    rel = (np.abs(rec - ref) / np.abs(ref)).max(axis=1)
+    print(f"device vs reference: steps 1-100 max rel {rel[:100].max():.2e}", flush=True)
+    assert rel[:100].max() <= 1e-9
+    er, ed = G.events(ref), G.events(rec)
+    assert len(er) >= 1 and len(ed) == len(er)
+    for a, b in zip(er, ed):
+        print(f"event onset {a[0]:.6e} / {b[0]:.6e} s, peak V {a[1]:.4f} / {b[1]:.4f} m/s", flush=True)
+        assert abs(a[0] - b[0]) <= 1e-3 * a[0]
+        assert abs(a[1] - b[1]) <= 1e-2 * a[1]
+    s.close()
