@@ -130,6 +130,16 @@ int pq3d_dense_step(void* sim, double dttry, double* h, double* dtnext, int* nrj
  * records (host, optional): nsteps x 4 = h, dtnext, time, max slip rate */
 int pq3d_dense_run(void* sim, long long nsteps, int graph_steps, double* records, int* status);
 int pq3d_dense_reset_ctrl(void* sim);
+/* Dilatancy / pore-pressure diffusion (QDsim_gpu.py:336-349, :611-671):
+ * zp K grid nodes, parr n x K initial profiles in Pa (row per element),
+ * c_hyd, eps (dilatancy coefficient), hs, beta (effective compressibility),
+ * P0 fixed far-field pressure (Pa), V0; every accepted step then updates the
+ * profiles, P and dPdt0 on the device (K <= 257) */
+int pq3d_dense_set_dilatancy(void* sim, int K, const double* zp, const double* parr, double c_hyd, double eps,
+                             double hs, double beta, double P0, double V0);
+int pq3d_dense_get_dilatancy(void* sim, double* parr);
+/* one pore-pressure update with step dt (Calc_P_implicit(dt), QDsim_gpu.py:628) */
+int pq3d_dense_dilatancy_update(void* sim, double dt);
 
 /* ------------------------------------------------------------------------
  * 5. Ensemble: B = 64 simulations on one mesh (no reference counterpart;

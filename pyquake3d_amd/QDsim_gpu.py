@@ -20,8 +20,8 @@ The Green's functions are assembled on the GPU in FP64 (or FP32 storage with
 the ``GF precision: fp32`` parameter key, which the reference does not have)
 and cached as ``A1s.npy ... Bd.npy`` in the ``Corefunc directory``, in the
 reference's layout (float64, row = receiver, column = source,
-QDsim_gpu.py:188-245).  Not supported: dilatancy / pore pressure (``If
-Dilatancy: True`` raises).
+QDsim_gpu.py:188-245).  Dilatancy / pore pressure (``If Dilatancy:
+True``) runs on the device after every accepted step (QDsim_gpu.py:611-671).
 """
 import ctypes
 import os
@@ -58,6 +58,9 @@ def _declare(L):
     _lib._sig(L, "pq3d_dense_step", [vp, d, dp, dp, ip])
     _lib._sig(L, "pq3d_dense_run", [vp, ll, i, dp, ip])
     _lib._sig(L, "pq3d_dense_reset_ctrl", [vp])
+    _lib._sig(L, "pq3d_dense_set_dilatancy", [vp, i, dp, dp, d, d, d, d, d, d])
+    _lib._sig(L, "pq3d_dense_get_dilatancy", [vp, dp])
+    _lib._sig(L, "pq3d_dense_dilatancy_update", [vp, d])
 
 
 _lib.register(_declare)
@@ -103,6 +106,28 @@ class DenseSim:
         check(L.pq3d_dense_matrix_ptrs(h, ptrs, ctypes.byref(ld)), "matrix_ptrs")
         self.mat_ptrs = [int(p) if p else 0 for p in ptrs]
         self.ld = ld.value
+
+    def set_dilatancy(self, zp, Parr, c_hyd, eps, hs, beta, P0, V0):
+        """Enable the pore-pressure update after every accepted step
+        (QDsim_gpu.py:628-671); ``Parr`` (n, K) profiles in Pa on the grid zp."""
+        zp = np.ascontiguousarray(zp, dtype=np.float64)
+        Parr = np.ascontiguousarray(Parr, dtype=np.float64)
+        if Parr.shape != (self.n, len(zp)):
+            raise ValueError("Parr must be (n, len(zp))")
+        check(_lib.lib().pq3d_dense_set_dilatancy(self.h, len(zp), _lib.ptr(zp), _lib.ptr(Parr), float(c_hyd),
+                                                  float(eps), float(hs), float(beta), float(P0), float(V0)),
+              "set_dilatancy")
+        self._dila_K = len(zp)
+
+    def dilatancy_update(self, dt):
+        """One pore-pressure update with step dt (Calc_P_implicit, QDsim_gpu.py:628)."""
+        check(_lib.lib().pq3d_dense_dilatancy_update(self.h, float(dt)), "dilatancy_update")
+
+    def get_dilatancy(self):
+        """Current pore-pressure profiles (n, K) in Pa."""
+        out = np.empty((self.n, self._dila_K))
+        check(_lib.lib().pq3d_dense_get_dilatancy(self.h, _lib.ptr(out)), "get_dilatancy")
+        return out
 
     def attach_hmatrix(self, hmat):
         """Take the traction rates from a ``Hmatrix.DeviceHmatrix`` (create the
@@ -268,9 +293,10 @@ class QDsim:
         self.InputHetoparamter = P["InputHetoparamter"] == "True"
         self.useGPU = P.get("GPU", "True") == "True"
         self.Ifdila = P.get("If Dilatancy", "False") == "True"
-        if self.Ifdila:
-            raise NotImplementedError("dilatancy / pore-pressure diffusion is not implemented "
-                                      "(QDsim_gpu.py:611-671)")
+        self.DilatancyC = float(P.get("Dilatancy coefficient", 0.0))
+        self.Chyd = float(P.get("Hydraulic diffusivity", 0.0))
+        self.hs = float(P.get("Actively shearing zone thickness", 0.0))
+        self.EPermeability = float(P.get("Effective compressibility", 0.0))
         prec = P.get("GF precision", "fp64").lower()
         self.mat_type = {"fp64": _lib.MAT_F64, "fp32": _lib.MAT_F32, "bf16": _lib.MAT_BF16}[prec]
         YoungsM = self.mu * (3.0 * self.lambda_ + 2.0 * self.mu) / (self.lambda_ + self.mu)
@@ -392,6 +418,14 @@ class QDsim:
         self.P = np.zeros(N)
         self.dPdt0 = np.zeros(N)
         self.local_index = np.arange(0, N, 1)
+        if self.Ifdila:
+            # pore-pressure grid and profiles (QDsim_gpu.py:336-349), pressures in Pa
+            num, c = 60, 1e-2
+            self.P0 = float(P["Constant porepressure"]) * 1e6
+            self.P = np.ones(N) * float(P["Initial porepressure"]) * 1e6
+            self.yp = np.logspace(start=-3, stop=10, num=num, base=10)
+            self.zp = np.log(1 + self.yp / c)
+            self.Parr = np.ones([N, num]) * float(P["Initial porepressure"]) * 1e6
         solve_normal = P["Normal traction solved from stress tensor"] == "True"
         depth_var = P["Vertical principal stress value varies with depth"] == "True"
         Ph1ang = np.pi / 180. * float(P["Angle between ssh1 and X-axis"])
@@ -535,6 +569,9 @@ class QDsim:
         s.set("rake0", self.rake)
         s.set_params(self.V0, self.Vpl_con, self.mu / (2.0 * self.Cs), self.fix_Tn, True)
         s.set_time(self.time)
+        if self.Ifdila:
+            s.set_dilatancy(self.zp, self.Parr, self.Chyd, self.DilatancyC, self.hs, self.EPermeability, self.P0,
+                            self.V0)
         self._gpu_ready = True
 
     def _step(self, dttry, gpu_semantics):
@@ -552,6 +589,16 @@ class QDsim:
         self.maxslipv0 = st["maxV"]
         return h, dtnext
 
+    def Calc_P_implicit(self, dt):
+        """One pore-pressure update on the device (QDsim_gpu.py:628-671); the
+        step drivers already run it after every accepted step."""
+        if not self.__dict__.get("_gpu_ready"):
+            self.initGPUvariable()
+        self.sim.dilatancy_update(dt)
+        self.P = self.sim.get("P")
+        self.dPdt0 = self.sim.get("dPdt0")
+        self.Parr = self.sim.get_dilatancy()
+
     def simu_forwardGPU(self, dttry):
         """One accepted step on the device; returns (h, dtnext)."""
         return self._step(dttry, True)
@@ -563,6 +610,10 @@ class QDsim:
         for name in ("Tno", "Tt1o", "Tt2o", "state", "slipv1", "slipv2", "slipv", "slip1", "slip2", "slip", "Tt",
                      "rake", "fric"):
             setattr(self, name, self.sim.get(name))
+        if self.Ifdila:
+            self.P = self.sim.get("P")
+            self.dPdt0 = self.sim.get("dPdt0")
+            self.Parr = self.sim.get_dilatancy()
         return h, dtnext
 
     def run(self, nsteps, dttry=None, graph_steps=8, records=True):

@@ -21,6 +21,7 @@
 #include "common.hpp"
 
 #include <algorithm>
+#include <cmath>
 #include <cstdint>
 #include <vector>
 
@@ -498,6 +499,72 @@ __global__ void pad_zero_kernel(double* p, long long n) {
 // ---------------------------------------------------------------------------
 // Host side: the dense simulation object
 // ---------------------------------------------------------------------------
+// ---------------------------------------------------------------------------
+// Dilatancy: pore-pressure diffusion normal to the fault, one backward-Euler
+// update per accepted step (QDsim_gpu.py:628-671).  Each element owns a
+// profile of K nodes on a stretched grid (node K-1 held at P0); the K-1
+// unknowns solve (I + dt P) x = b with P tridiagonal and common to all
+// elements, so one workgroup-wide elimination of the matrix (LDS) serves
+// every element's forward/backward sweep.  Profiles are node-major [K][n].
+// ---------------------------------------------------------------------------
+constexpr int DILA_MAX_NODES = 256;
+
+struct DilaConst {
+    double c_hyd, eps, hs, beta, P0;
+    double qb0;     // 2 c_hyd exp(-(z0 - dz0/2)) / dz0: flux source at node 0
+    double qend;    // c_hyd exp(-(z[K-2] - dz[K-2]/2)) / dz[K-2]^2: the fixed boundary
+    double e_lo0, e_hi0, ez0, emz0, dz0;
+    double V0;
+    int K;
+};
+
+// dt_given >= 0: update with that step size (the reference's
+// Calc_P_implicit(dt)); otherwise with the last accepted step, unless it failed
+__global__ __launch_bounds__(256) void dila_kernel(double* __restrict__ F, long long n,
+                                                   const Ctrl* __restrict__ ctrl, double* __restrict__ parr,
+                                                   const double* __restrict__ tri, DilaConst c, double dt_given) {
+    __shared__ double s_cp[DILA_MAX_NODES], s_inv[DILA_MAX_NODES];
+    const int m = c.K - 1;
+    const double dt = dt_given >= 0.0 ? dt_given : ctrl->last_h;
+    const bool ok = dt_given >= 0.0 || ctrl->status == 0;
+    if (threadIdx.x == 0 && ok) {
+        double cp = 0.0;
+        for (int i = 0; i < m; ++i) {
+            const double lo = i > 0 ? dt * tri[i] : 0.0;
+            const double up = i < m - 1 ? dt * tri[2 * m + i] : 0.0;
+            const double den = (1.0 + dt * tri[m + i]) - lo * cp;
+            s_inv[i] = 1.0 / den;
+            cp = up / den;
+            s_cp[i] = cp;
+        }
+    }
+    __syncthreads();
+    const long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (!ok || e >= n) return;
+    const double theta = F[F_DC * n + e] / c.V0 * exp((F[F_STATE * n + e] - F[F_F0 * n + e]) / F[F_B * n + e]);
+    const double dthetadt = 1.0 - F[F_SLIPV * n + e] * theta / F[F_DC * n + e];
+    const double g = -c.eps * c.hs / (2.0 * c.beta * c.c_hyd * theta) * dthetadt;
+    double d = 0.0;
+    for (int i = 0; i < m; ++i) {
+        double b = parr[(size_t)i * n + e];
+        if (i == 0) b -= c.qb0 * g * dt;
+        if (i == m - 1) b += dt * c.qend * c.P0;
+        const double lo = i > 0 ? dt * tri[i] : 0.0;
+        d = (b - lo * d) * s_inv[i];
+        parr[(size_t)i * n + e] = d;
+    }
+    double x = d, x1 = d;
+    for (int i = m - 2; i >= 0; --i) {
+        x1 = x;
+        x = parr[(size_t)i * n + e] - s_cp[i] * x;
+        parr[(size_t)i * n + e] = x;
+    }
+    const double term1 = -c.e_lo0 * (x - x1 + 2 * c.dz0 * g * c.ez0);
+    const double term2 = c.e_hi0 * (x1 - x);
+    F[F_DPDT0 * n + e] = c.c_hyd * c.emz0 * (term1 + term2) / (c.dz0 * c.dz0);
+    F[F_P * n + e] = x;
+}
+
 // H-matrix traction rates (hmat_mvm.hip): out = 3 x n element-order sums
 void hmat_launch(void* hm, const double* V1, const double* V2, const double* c0, const double* s0, double Vpl,
                  int fix_tn, double* out, hipStream_t st);
@@ -505,6 +572,10 @@ long long hmat_n(void* hm);
 
 struct DenseSim {
     void* hmat = nullptr;          // attached H-matrix (replaces the dense matrices)
+    int dila_on = 0;               // pore-pressure update after every accepted step
+    double* dila_parr = nullptr;   // K x n profiles (Pa)
+    double* dila_tri = nullptr;    // 3 x (K-1): lower, diagonal, upper of the operator
+    DilaConst dila;
     long long n = 0, ld = 0;
     int mat_type = MAT_F32;
     void* mats[6] = {nullptr};
@@ -584,6 +655,9 @@ static void launch_step(DenseSim* s, hipStream_t st, double* records, long long 
     launch_matvec(s, st);
     SimArgs a = make_args(s, records, rec_index);
     hipLaunchKernelGGL(rk_step_kernel, dim3((unsigned)s->nblocks), dim3(256), 0, st, a);
+    if (s->dila_on)
+        hipLaunchKernelGGL(dila_kernel, dim3((unsigned)((s->n + 255) / 256)), dim3(256), 0, st, s->fields, s->n,
+                           (const Ctrl*)s->ctrl, s->dila_parr, (const double*)s->dila_tri, s->dila, -1.0);
 }
 
 static int configure(DenseSim* s) {
@@ -655,7 +729,100 @@ int pq3d_dense_destroy(void* h) {
     (void)hipFree(s->partial);
     (void)hipFree(s->ctrl);
     (void)hipFree(s->d_records);
+    (void)hipFree(s->dila_parr);
+    (void)hipFree(s->dila_tri);
     delete s;
+    return 0;
+}
+
+// Dilatancy / pore-pressure diffusion (QDsim_gpu.py:336-349, :611-671).
+// zp: K node coordinates of the stretched grid; parr: n x K initial profiles
+// (Pa, row per element, the reference's Parr layout); c_hyd hydraulic
+// diffusivity, eps dilatancy coefficient, hs shear-zone thickness, beta
+// effective compressibility, P0 fixed far-field pressure (Pa), V0 reference
+// slip rate.  From the next step on, every accepted step updates the
+// profiles, P and dP/dt.
+int pq3d_dense_set_dilatancy(void* h, int K, const double* zp, const double* parr, double c_hyd, double eps,
+                             double hs, double beta, double P0, double V0) {
+    clear_error();
+    DenseSim* s = (DenseSim*)h;
+    PQ_CHECK(s && K >= 3 && K <= DILA_MAX_NODES + 1, "pq3d_dense_set_dilatancy: K must be in [3, %d]",
+             DILA_MAX_NODES + 1);
+    const int m = K - 1;
+    std::vector<double> dz(m), tri(3 * (size_t)m, 0.0);
+    for (int i = 0; i < m; ++i) dz[i] = zp[i + 1] - zp[i];
+    for (int i = 0; i < m; ++i) {
+        const double lo = std::exp(-(2 * zp[i] - dz[i] / 2)), hi = std::exp(-(2 * zp[i] + dz[i] / 2));
+        const double w = c_hyd / (dz[i] * dz[i]);
+        tri[m + i] = w * (lo + hi);                              // diagonal
+        if (i == 0) {
+            tri[2 * m + 0] = -w * (lo + hi);                     // row 0 couples node 1 with both weights
+        } else {
+            tri[i] = -w * lo;                                    // lower
+            if (i < m - 1) tri[2 * m + i] = -w * hi;             // upper (node K-1 is the fixed boundary)
+        }
+    }
+    DilaConst c;
+    c.c_hyd = c_hyd;
+    c.eps = eps;
+    c.hs = hs;
+    c.beta = beta;
+    c.P0 = P0;
+    c.dz0 = dz[0];
+    c.qb0 = 2.0 * c_hyd * std::exp(-(zp[0] - dz[0] / 2)) / dz[0];
+    c.qend = c_hyd * std::exp(-(zp[m - 1] - dz[m - 1] / 2)) / (dz[m - 1] * dz[m - 1]);
+    c.e_lo0 = std::exp(-(zp[0] - dz[0] / 2));
+    c.e_hi0 = std::exp(-(zp[0] + dz[0] / 2));
+    c.ez0 = std::exp(zp[0]);
+    c.emz0 = std::exp(-zp[0]);
+    c.V0 = V0;
+    c.K = K;
+    const long long n = s->n;
+    std::vector<double> nm((size_t)K * n);
+    for (long long e = 0; e < n; ++e)
+        for (int k = 0; k < K; ++k) nm[(size_t)k * n + e] = parr[(size_t)e * K + k];
+    if (s->dila_parr) (void)hipFree(s->dila_parr);
+    if (s->dila_tri) (void)hipFree(s->dila_tri);
+    s->dila_parr = nullptr;
+    s->dila_tri = nullptr;
+    PQ_HIP(hipMalloc((void**)&s->dila_parr, nm.size() * sizeof(double)));
+    PQ_HIP(hipMalloc((void**)&s->dila_tri, tri.size() * sizeof(double)));
+    PQ_HIP(hipMemcpy(s->dila_parr, nm.data(), nm.size() * sizeof(double), hipMemcpyHostToDevice));
+    PQ_HIP(hipMemcpy(s->dila_tri, tri.data(), tri.size() * sizeof(double), hipMemcpyHostToDevice));
+    s->dila = c;
+    s->dila_on = 1;
+    if (s->graph) {
+        (void)hipGraphExecDestroy(s->graph);
+        s->graph = nullptr;
+    }
+    return 0;
+}
+
+// One pore-pressure update with step size dt from the current state and slip
+// rates (the reference's Calc_P_implicit(dt), QDsim_gpu.py:628-671).
+int pq3d_dense_dilatancy_update(void* h, double dt) {
+    clear_error();
+    DenseSim* s = (DenseSim*)h;
+    PQ_CHECK(s && s->dila_on, "pq3d_dense_dilatancy_update: dilatancy is not enabled");
+    PQ_CHECK(dt >= 0.0, "pq3d_dense_dilatancy_update: dt must be >= 0");
+    hipLaunchKernelGGL(dila_kernel, dim3((unsigned)((s->n + 255) / 256)), dim3(256), 0, 0, s->fields, s->n,
+                       (const Ctrl*)s->ctrl, s->dila_parr, (const double*)s->dila_tri, s->dila, dt);
+    PQ_HIP(hipGetLastError());
+    PQ_HIP(hipDeviceSynchronize());
+    return 0;
+}
+
+// Current pore-pressure profiles into parr (n x K, row per element).
+int pq3d_dense_get_dilatancy(void* h, double* parr) {
+    clear_error();
+    DenseSim* s = (DenseSim*)h;
+    PQ_CHECK(s && s->dila_on, "pq3d_dense_get_dilatancy: dilatancy is not enabled");
+    const int K = s->dila.K;
+    const long long n = s->n;
+    std::vector<double> nm((size_t)K * n);
+    PQ_HIP(hipMemcpy(nm.data(), s->dila_parr, nm.size() * sizeof(double), hipMemcpyDeviceToHost));
+    for (long long e = 0; e < n; ++e)
+        for (int k = 0; k < K; ++k) parr[(size_t)e * K + k] = nm[(size_t)k * n + e];
     return 0;
 }
 

@@ -1,0 +1,134 @@
+"""GPU tests of the pore-pressure (dilatancy) update.
+
+* One device update against the reference's own Calc_P_implicit vectors
+  (tests/golden/dilatancy_golden.npz): 40 elements, dt = 1e-3, 1e5, 3e7 s,
+  two consecutive updates each; profiles and P within 1e-12; dP/dt within
+  1e-11 of the scale of the flux terms it is the difference of,
+  c_hyd exp(-z0) exp(-(z0 - dz0/2)) max|P| / dz0^2 (the device eliminates the
+  tridiagonal system without pivoting; the reference uses LAPACK's pivoted LU).
+* 30 coupled steps (RSF + radiation damping + pore pressure after every
+  accepted step) against the NumPy oracle doing the same: step sizes within
+  1e-9, P, the profiles and the state within 1e-9 of their scale, dP/dt
+  within 1e-9 of the flux-term scale.
+"""
+import os
+
+import numpy as np
+import pytest
+
+from oracle import dilatancy_oracle as D
+from oracle import qdsim_oracle as Q
+
+pytestmark = pytest.mark.gpu
+
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "dilatancy_golden.npz")
+MU = LAM = 32038120320.0
+
+
+def flux_scale(zp, c_hyd, Parr):
+    """Scale of the terms whose difference is dP/dt (QDsim_gpu.py:662-664)."""
+    dz0 = zp[1] - zp[0]
+    return c_hyd * np.exp(-zp[0]) * np.exp(-(zp[0] - dz0 / 2)) * np.abs(Parr).max() / dz0 ** 2
+CS = 3464.0
+V0, VPL = 1e-6, 1e-9
+
+
+def test_update_matches_reference_vectors():
+    from pyquake3d_amd import _lib
+    from pyquake3d_amd.QDsim_gpu import DenseSim
+    g = dict(np.load(GOLDEN, allow_pickle=False))
+    zp = g["zp"]
+    for c in range(3):
+        n = len(g[f"c{c}_state"])
+        s = DenseSim(n, _lib.MAT_F32)
+        for name in ("state", "slipv", "dc", "f0", "b"):
+            s.set(name, g[f"c{c}_{name}"])
+        s.set_dilatancy(zp, g[f"c{c}_Parr_in0"], float(g["Chyd"]), float(g["DilatancyC"]), float(g["hs"]),
+                        float(g["EPermeability"]), float(g["P0"]), float(g["V0"]))
+        for it in range(2):
+            s.dilatancy_update(float(g[f"c{c}_dt"]))
+            Parr = s.get_dilatancy()
+            want = g[f"c{c}_Parr_out{it}"]
+            assert np.abs(Parr - want).max() <= 1e-12 * np.abs(want).max(), (c, it)
+            assert np.abs(s.get("P") - g[f"c{c}_P_out{it}"]).max() <= 1e-12 * np.abs(want).max()
+            dp = g[f"c{c}_dPdt0_out{it}"]
+            assert np.abs(s.get("dPdt0") - dp).max() <= 1e-11 * flux_scale(zp, float(g["Chyd"]), want), (c, it)
+        s.close()
+
+
+def test_coupled_steps_follow_oracle():
+    from pyquake3d_amd import _lib, greenfunc, readmsh, synth
+    from pyquake3d_amd.QDsim_gpu import DenseSim
+    nodes_km, tri = synth.planar_fault_mesh(20, 10, 40.0, 20.0)
+    nodes = nodes_km * 1e3
+    ev, xg = readmsh.get_eleVec(nodes, tri, False)
+    n = len(tri)
+    cols = synth.friction_params(xg, dc_sigma=0.2, seed=3, rake=0.5)
+    s = DenseSim(n, _lib.MAT_F64)
+    s.assemble(*greenfunc.source_vertices(nodes, tri), xg, ev, MU, LAM, True)
+    mats = [s.download_matrix(k) for k in range(6)]
+    Pinit, P0 = 2.0e6, 2.0e6
+    # steady state with the effective normal stress (QDsim_gpu.py:750-759)
+    rake, a, b, dc, f0, _, Tn, V, sl, nl = cols.T
+    orc = Q.from_init_columns(mats, cols, V0, VPL, MU, CS)
+    Tt, state = Q.steady_state(a, b, f0, Tn, V, V0, VPL, P=Pinit)
+    orc.Tt1o, orc.Tt2o, orc.state = Tt * np.cos(rake), Tt * np.sin(rake), state
+    orc.P = np.full(n, Pinit)
+    zp = D.pore_grid()
+    Pm = D.pmatrix(zp, 1e-5)
+    Parr = np.full((n, len(zp)), Pinit)
+    prm = dict(e=3e-4, hs=3e-3, beta=8e-11, c_hyd=1e-5)
+    for name in ("Tno", "Tt1o", "Tt2o", "state", "slipv1", "slipv2", "slipv", "a", "b", "dc", "f0",
+                 "shear_loading", "normal_loading", "P", "dPdt0"):
+        s.set(name, getattr(orc, name))
+    s.set("rake0", orc.rake0)
+    s.set_params(V0, VPL, MU / (2 * CS), False, True)
+    s.set_dilatancy(zp, Parr, prm["c_hyd"], prm["e"], prm["hs"], prm["beta"], P0, V0)
+    dt = do = 1e-3
+    for k in range(30):
+        h, dt, _ = s.step(dt)
+        ho, do = orc.step(do)
+        assert h is not None and abs(h - ho) <= 1e-9 * ho, (k, h, ho)
+        g = D.source_g(orc.state, orc.slipv, orc.dc, orc.f0, orc.b, V0, prm["e"], prm["hs"], prm["beta"],
+                       prm["c_hyd"])
+        Parr, orc.P, orc.dPdt0 = D.implicit_step(Parr, g, zp, Pm, prm["c_hyd"], P0, ho)
+    assert np.abs(s.get("P") - orc.P).max() <= 1e-9 * np.abs(orc.P).max()
+    assert np.abs(s.get("dPdt0") - orc.dPdt0).max() <= 1e-9 * flux_scale(zp, prm["c_hyd"], Parr)
+    assert np.abs(s.get("state") - orc.state).max() <= 1e-9 * np.abs(orc.state).max()
+    assert np.abs(s.get_dilatancy() - Parr).max() <= 1e-9 * np.abs(Parr).max()
+    s.close()
+
+
+def test_dropin_qdsim_with_dilatancy(tmp_path):
+    """The reference's driver pattern with ``If Dilatancy: True``: the mirror
+    builds the pore-pressure grid and profiles from the parameter file
+    (QDsim_gpu.py:336-349) and the device updates them after every accepted
+    step; the run follows the oracle doing the same (step sizes, P)."""
+    from pyquake3d_amd import QDsim_gpu, readmsh, synth
+    msh, par = synth.write_case(str(tmp_path / "case"), 12, 6, 24.0, 12.0, dc_sigma=0.2, seed=1)
+    txt = open(par).read().replace("If Dilatancy:False", "If Dilatancy:True")
+    open(par, "w").write(txt)
+    nodelst, elelst = readmsh.read_mshV2(msh)
+    sim0 = QDsim_gpu.QDsim(elelst, nodelst, par)
+    assert sim0.Ifdila and sim0.Parr.shape == (len(elelst), 60)
+    core = sim0.Para0["Corefunc directory"]
+    mats = [np.load(os.path.join(core, n + ".npy")) for n in ("A1s", "A2s", "Bs", "A1d", "A2d", "Bd")]
+    orc = Q.from_init_columns(mats, np.loadtxt(os.path.join(os.path.dirname(par), "param.dat")), V0, VPL, MU,
+                              np.sqrt(MU / sim0.density))
+    for name in ("Tt1o", "Tt2o", "state", "P"):
+        setattr(orc, name, np.array(getattr(sim0, name), dtype=np.float64))
+    zp, Pm = sim0.zp, D.pmatrix(sim0.zp, sim0.Chyd)
+    Parr = sim0.Parr.copy()
+    sim0.initGPUvariable()
+    dttry = sim0.htry
+    dto = dttry
+    for i in range(20):
+        h, dtnext = sim0.simu_forward(dttry)
+        ho, dto = orc.step(dto)
+        assert abs(h - ho) <= 1e-9 * ho, (i, h, ho)
+        g = D.source_g(orc.state, orc.slipv, orc.dc, orc.f0, orc.b, sim0.V0, sim0.DilatancyC, sim0.hs,
+                       sim0.EPermeability, sim0.Chyd)
+        Parr, orc.P, orc.dPdt0 = D.implicit_step(Parr, g, zp, Pm, sim0.Chyd, sim0.P0, ho)
+        dttry = dtnext
+    assert np.abs(sim0.P - orc.P).max() <= 1e-9 * np.abs(orc.P).max()
+    assert np.abs(sim0.Parr - Parr).max() <= 1e-9 * np.abs(Parr).max()
