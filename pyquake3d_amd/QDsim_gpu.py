@@ -589,6 +589,44 @@ class QDsim:
         self.maxslipv0 = st["maxV"]
         return h, dtnext
 
+    # (field name in the file, attribute, scale, number format): the cell data
+    # of the reference's legacy VTK output, in its order (QDsim_gpu.py:1927-2073)
+    _VTK_FIELDS = (("Normal_[MPa]", "Tno", 1.0, "%f"), ("Pore_pressure[MPa]", "P", 1e-6, "%f"),
+                   ("Shear_[MPa]", "Tt", 1.0, "%f"), ("Shear_1[MPa]", "Tt1o", 1.0, "%f"),
+                   ("Shear_2[MPa]", "Tt2o", 1.0, "%f"), ("rake[Degree]", "rake", 180. / np.pi, "%f"),
+                   ("state", "state", 1.0, "%f"), ("Slipv[m/s]", "slipv", 1.0, "%.20f"),
+                   ("Slipv1[m/s]", "slipv1", 1.0, "%.20f"), ("Slipv2[m/s]", "slipv2", 1.0, "%.20f"),
+                   ("a", "a", 1.0, "%f"), ("b", "b", 1.0, "%f"), ("a-b", None, 1.0, "%f"), ("dc", "dc", 1.0, "%f"),
+                   ("fric", "fric", 1.0, "%f"), ("slip", "slip", 1.0, "%.20f"), ("slip1", "slip1", 1.0, "%.20f"),
+                   ("slip2", "slip2", 1.0, "%.20f"))
+
+    def ouputVTK(self, fname):
+        """Legacy-VTK snapshot of the fault in the reference's layout
+        (QDsim_gpu.py:1927-2073): nodes, triangles, and per-cell fields.  The
+        values are the current device state (the reference's GPU driver
+        writes some fields from host copies it does not refresh)."""
+        if self.__dict__.get("_gpu_ready"):
+            for name in ("Tno", "P", "Tt", "Tt1o", "Tt2o", "rake", "state", "slipv", "slipv1", "slipv2", "fric",
+                         "slip", "slip1", "slip2"):
+                setattr(self, name, self.sim.get(name))
+        nodes = np.asarray(self.nodelst, dtype=np.float64)
+        tri = np.asarray(self.elelst, dtype=np.int64) - 1
+        ne = len(tri)
+        with open(fname, "w") as f:
+            f.write("# vtk DataFile Version 3.0\ntest\nASCII\nDATASET  UNSTRUCTURED_GRID\n")
+            f.write("POINTS %d float\n" % len(nodes))
+            f.write("".join("%f %f %f\n" % tuple(p) for p in nodes))
+            f.write("CELLS %d %d\n" % (ne, ne * 4))
+            f.write("".join("3 %d %d %d\n" % tuple(t) for t in tri))
+            f.write("CELL_TYPES %d\n" % ne)
+            f.write("5 " * ne + "\n")
+            f.write("CELL_DATA %d " % ne)
+            for label, attr, scale, fmt in self._VTK_FIELDS:
+                vals = (np.asarray(self.a) - np.asarray(self.b)) if attr is None else \
+                    np.asarray(getattr(self, attr), dtype=np.float64) * scale
+                f.write("SCALARS %s float\nLOOKUP_TABLE default\n" % label)
+                f.write("".join((fmt + " ") % v for v in vals) + "\n")
+
     def Calc_P_implicit(self, dt):
         """One pore-pressure update on the device (QDsim_gpu.py:628-671); the
         step drivers already run it after every accepted step."""

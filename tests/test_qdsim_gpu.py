@@ -199,6 +199,19 @@ def test_dropin_qdsim_api(tmp_path):
         assert np.abs(v1 - orc.slipv1).max() <= 1e-9 * np.abs(orc.slipv1).max()
         dttry = dtnext
     assert abs(sim0.time - orc.time) <= 1e-9 * orc.time
+    # the reference's legacy VTK snapshot: layout, counts and current values
+    vtk = str(tmp_path / "step.vtk")
+    sim0.ouputVTK(vtk)
+    lines = open(vtk).read().split("\n")
+    assert lines[0] == "# vtk DataFile Version 3.0" and lines[3] == "DATASET  UNSTRUCTURED_GRID"
+    assert lines[4] == "POINTS %d float" % len(nodelst)
+    n = len(elelst)
+    i = lines.index("SCALARS Slipv1[m/s] float")
+    v1 = np.array(lines[i + 2].split(), dtype=np.float64)
+    assert len(v1) == n and np.allclose(v1, sim0.slipv1_gpu.get(), rtol=0, atol=1e-20)
+    import re
+    labels = re.findall(r"SCALARS (\S+) float", "\n".join(lines))  # the first shares the CELL_DATA line
+    assert labels == [f[0] for f in QDsim_gpu.QDsim._VTK_FIELDS]
     # a second construction loads the cached Green's functions
     sim1 = QDsim_gpu.QDsim(elelst, nodelst, par)
     assert np.array_equal(sim1.A1s, sim0.A1s)
