@@ -363,14 +363,59 @@ def hmatrix_leg(steps=200, warmup=10, graph_steps=8, cpu_budget_s=20.0):
     return res
 
 
-def _pmc_traffic(name):
-    """HBM bytes per launch from a committed rocprofv3 --pmc summary, or None."""
+def _tuned(obj, res, builtin_kc):
+    """Autotuner result for the JSON line: the chosen split-K width and its
+    time beside the built-in width's (both from the same sweep)."""
+    table = dict(res["table"])
+    kc_b = min(table, key=lambda k: abs(k - builtin_kc))
+    log(f"[bench] autotune {res['key'].split('|')[2]}: kc {res['kc']} ({res['ms']:.3f} ms per product), "
+        f"built-in kc {builtin_kc}: {table[kc_b]:.3f} ms, {len(table)} candidates")
+    return {"kc": res["kc"], "ms_per_product": res["ms"], "builtin_kc": builtin_kc,
+            "builtin_ms_per_product": table[kc_b], "candidates": len(table),
+            "table": [[k, round(v, 5)] for k, v in res["table"]]}
+
+
+def _pmc_json(name):
+    """A committed rocprofv3 --pmc summary (profiles/<name>), or None."""
     p = os.path.join(ROOT, "profiles", name)
     try:
         with open(p) as f:
-            return float(json.load(f)["hbm_bytes_per_launch"])
-    except (OSError, ValueError, KeyError):
+            return json.load(f)
+    except (OSError, ValueError):
         return None
+
+
+def _pmc_traffic(name):
+    """HBM bytes per launch from a committed rocprofv3 --pmc summary, or None."""
+    d = _pmc_json(name)
+    try:
+        return float(d["hbm_bytes_per_launch"]) if d else None
+    except (KeyError, TypeError, ValueError):
+        return None
+
+
+def _pmc_mfma(name):
+    """MFMA busy fraction and counted MFMA flops of the ensemble GEMM from the
+    committed counter summary (SQ_VALU_MFMA_BUSY_CYCLES, SQ_INSTS_VALU_MFMA_MOPS_F32)."""
+    d = _pmc_json(name)
+    if not d:
+        return None
+    return {"mfma_util": d.get("mfma_util"), "mfma_flops_per_launch": d.get("mfma_flops_per_launch"),
+            "mfma_flops_over_algorithmic": d.get("mfma_flops_over_algorithmic"), "source": "profiles/" + name}
+
+
+def _pmc_fp64(name, pairs, ms):
+    """Hardware-counted FP64 flops of the assembly kernel ((2 FMA + ADD + MUL) x 64
+    per wave instruction, committed counter summary) at this run's timing."""
+    d = _pmc_json(name)
+    try:
+        k = d["per_output_type"]["assemble_f32out"]
+        fpp = float(k["fp64_flops_per_pair"])
+    except (KeyError, TypeError, ValueError):
+        return None
+    tf = pairs * fpp / (ms * 1e-3) / 1e12
+    return {"fp64_flops_per_pair": fpp, "fp64_tflops": tf, "frac_fp64_peak": tf / FP64_PEAK_TFLOPS,
+            "valu_issue_util": k.get("valu_issue_util"), "source": "profiles/" + name}
 
 
 def main():
@@ -410,7 +455,7 @@ def main():
                                      cpu_budget_s=0.0 if args.no_cpu_baseline else 20.0)))
         return
 
-    from pyquake3d_amd import _lib, synth
+    from pyquake3d_amd import _lib, autotune, synth
     from pyquake3d_amd.QDsim_gpu import DenseSim
     from pyquake3d_amd.ensemble import Ensemble, init_members
 
@@ -433,11 +478,14 @@ def main():
     if fpp:
         tf = pairs * fpp / (asm_ms * 1e-3) / 1e12
         gf.update({"fp64_flops_per_pair": fpp, "fp64_tflops": tf, "fp64_peak_tflops": FP64_PEAK_TFLOPS,
-                   "frac_fp64_peak": tf / FP64_PEAK_TFLOPS})
+                   "frac_fp64_peak": tf / FP64_PEAK_TFLOPS,
+                   "flop_convention": "source-level count, every add/sub/mul/div/sqrt/log/atan = 1"})
+    gf["pmc_counted"] = _pmc_fp64("gf_assemble_20k_fp64_pmc.json", pairs, asm_ms)
     log(f"[bench] assembly {asm_ms:.1f} ms, {gf['pairs_per_s']:.3e} pairs/s")
 
     # --- plain single simulation, FP32 storage
     init_state(sim, cols)
+    tuning = {"matvec_f32": _tuned(sim, autotune.tune_dense(sim, use_cache=False), 256)}
     mv_ms = sim.time_matvec(20)
     p32_ms = time_plain(sim, args.plain_steps, args.warmup, args.graph_steps)
     plain32 = args.plain_steps / (p32_ms * 1e-3)
@@ -449,6 +497,8 @@ def main():
     mcols = np.stack([synth.member_params(xg, m) for m in range(MEMBERS)])
     ens = Ensemble(N, MEMBERS, matrices_from=sim)
     init_members(ens, mcols, V0, VPL, MU, CS)
+    kc0 = ens.kc
+    tuning["ens_gemm"] = _tuned(ens, autotune.tune_ensemble(ens, use_cache=False), kc0)
     ens_ms, accepted = time_ensemble(ens, args.steps, args.warmup, args.graph_steps)
     value = accepted / (ens_ms * 1e-3)
     gemm_ms = ens.gemm(reps=10)
@@ -467,6 +517,7 @@ def main():
     s64 = DenseSim(N, _lib.MAT_F64)
     s64.assemble(*P, xg, ev, MU, LAM, True)
     init_state(s64, cols)
+    tuning["matvec_f64"] = _tuned(s64, autotune.tune_dense(s64, use_cache=False), 256)
     k64 = max(100, args.plain_steps // 4)
     p64_ms = time_plain(s64, k64, args.warmup, args.graph_steps)
     plain64 = k64 / (p64_ms * 1e-3)
@@ -519,7 +570,8 @@ def main():
         "roofline": {"bound": "mfma", "achieved": gemm_tf, "peak": FP32_MFMA_PEAK_TFLOPS, "unit": "TFLOP/s",
                      "frac": gemm_tf / FP32_MFMA_PEAK_TFLOPS, "traffic": _pmc_traffic("ens_gemm_20k_traffic.json"),
                      "kernel": "ens_gemm_kernel (v_mfma_f32_16x16x4f32, 6 x 20k x 20k FP32 by 20k x 64)",
-                     "flops_per_launch": gemm_flops, "avg_launch_ms": gemm_ms},
+                     "flops_per_launch": gemm_flops, "avg_launch_ms": gemm_ms,
+                     "pmc": _pmc_mfma("ens_gemm_20k_mfma_pmc.json")},
         "cpu_baseline": cpu,
         "plain_fp32_member_steps_per_s": plain32,
         "plain_fp64_member_steps_per_s": plain64,
@@ -530,6 +582,7 @@ def main():
                            "kernel": "matvec_kernel<float> (six FP32 traction matrices, FP64 accumulation)",
                            "alg_bytes_per_launch": mv_alg, "avg_launch_ms": mv_ms},
         "ensemble": ens_info,
+        "autotune": tuning,
         "gf_assembly": gf,
         "p99_solve_latency_s": serving["p99_time_to_result_s"] if serving else None,
         "serving": serving,

@@ -123,6 +123,13 @@ int pq3d_dense_state(void* sim, double* out6, int* nrjct, int* status, long long
 /* traction rates from the current slip rates into AdotV1/AdotV2/dsigmadt */
 int pq3d_dense_matvec(void* sim);
 int pq3d_dense_time_matvec(void* sim, int reps, float* ms_per_launch);
+/* split-K tile width of the dense product (source columns per tile; no
+ * reference counterpart: the reference's product is one CuPy matmul).
+ * set_kc waits for the device and reallocates the partial sums; time_product
+ * times product + partial-sum pass, as the autotuner compares widths */
+int pq3d_dense_set_kc(void* sim, int kc);
+int pq3d_dense_get_kc(void* sim, int* kc, int* nchunk);
+int pq3d_dense_time_product(void* sim, int reps, float* ms_per_product);
 /* one accepted step; returns 1 with 'error: dt is too small' in
  * pq3d_last_error() where the reference exits (QDsim_gpu.py:983-985) */
 int pq3d_dense_step(void* sim, double dttry, double* h, double* dtnext, int* nrjct);
@@ -168,6 +175,12 @@ int pq3d_ens_members(void* ens, double* d6, long long* i6);
 int pq3d_ens_reset_member(void* ens, int member, double h0, double t_end, int active);
 int pq3d_ens_gemm(void* ens, int reps, float* ms_per_launch);
 int pq3d_ens_gemm_result(void* ens, double* host_3Bn);
+/* split-K tile width of the GEMM (64 receivers x 64 members x kc sources per
+ * wave; kc rounded up to a multiple of 4); time_product = operand prep + GEMM
+ * + one pass over the partial sums */
+int pq3d_ens_set_kc(void* ens, int kc);
+int pq3d_ens_get_kc(void* ens, int* kc, int* nchunk);
+int pq3d_ens_time_product(void* ens, int reps, float* ms_per_product);
 /* enqueue nsteps global steps (HIP graph of graph_steps); async != 0 returns
  * without waiting */
 int pq3d_ens_run(void* ens, long long nsteps, int graph_steps, int async);

@@ -31,7 +31,7 @@ from math import sqrt
 
 import numpy as np
 
-from . import _lib, greenfunc, readmsh
+from . import _lib, autotune, greenfunc, readmsh
 from ._lib import check
 
 FIELDS = ("Tno", "Tt1o", "Tt2o", "state", "slipv1", "slipv2", "slipv", "slip1", "slip2", "slip", "Tt", "rake",
@@ -55,6 +55,9 @@ def _declare(L):
     _lib._sig(L, "pq3d_dense_state", [vp, dp, ip, ip, ctypes.POINTER(ll)])
     _lib._sig(L, "pq3d_dense_matvec", [vp])
     _lib._sig(L, "pq3d_dense_time_matvec", [vp, i, ctypes.POINTER(ctypes.c_float)])
+    _lib._sig(L, "pq3d_dense_time_product", [vp, i, ctypes.POINTER(ctypes.c_float)])
+    _lib._sig(L, "pq3d_dense_set_kc", [vp, i])
+    _lib._sig(L, "pq3d_dense_get_kc", [vp, ip, ip])
     _lib._sig(L, "pq3d_dense_step", [vp, d, dp, dp, ip])
     _lib._sig(L, "pq3d_dense_run", [vp, ll, i, dp, ip])
     _lib._sig(L, "pq3d_dense_reset_ctrl", [vp])
@@ -101,6 +104,7 @@ class DenseSim:
         self.h = h
         self.n = int(n)
         self.mat_type = mat_type
+        self.fix_tn = False
         ptrs = (ctypes.c_void_p * 6)()
         ld = ctypes.c_longlong()
         check(L.pq3d_dense_matrix_ptrs(h, ptrs, ctypes.byref(ld)), "matrix_ptrs")
@@ -144,7 +148,14 @@ class DenseSim:
         check(_lib.lib().pq3d_dense_get_field(self.h, FIELD_ID[name], _lib.ptr(out)), "get_field " + name)
         return out
 
+    @property
+    def has_dense(self):
+        """True when the traction product runs on this simulation's own dense
+        matrices (not MAT_NONE, no H-matrix attached)."""
+        return self.mat_type != _lib.MAT_NONE and "_hmat" not in self.__dict__
+
     def set_params(self, V0, Vpl, eta, fix_tn=False, gpu_semantics=True):
+        self.fix_tn = bool(fix_tn)
         check(_lib.lib().pq3d_dense_set_params(self.h, V0, Vpl, eta, int(bool(fix_tn)), int(bool(gpu_semantics))),
               "set_params")
 
@@ -195,6 +206,22 @@ class DenseSim:
         f = ctypes.c_float()
         check(_lib.lib().pq3d_dense_time_matvec(self.h, int(reps), ctypes.byref(f)), "time_matvec")
         return f.value
+
+    def time_product(self, reps=5):
+        """ms per traction product including the split-K partial-sum pass."""
+        f = ctypes.c_float()
+        check(_lib.lib().pq3d_dense_time_product(self.h, int(reps), ctypes.byref(f)), "time_product")
+        return f.value
+
+    @property
+    def kc(self):
+        """Split-K width of the dense product (source columns per tile)."""
+        kc, nc = ctypes.c_int(), ctypes.c_int()
+        check(_lib.lib().pq3d_dense_get_kc(self.h, ctypes.byref(kc), ctypes.byref(nc)), "get_kc")
+        return kc.value
+
+    def set_kc(self, kc):
+        check(_lib.lib().pq3d_dense_set_kc(self.h, int(kc)), "set_kc")
 
     def step(self, dttry):
         h = ctypes.c_double()
@@ -569,6 +596,8 @@ class QDsim:
         s.set("rake0", self.rake)
         s.set_params(self.V0, self.Vpl_con, self.mu / (2.0 * self.Cs), self.fix_Tn, True)
         s.set_time(self.time)
+        if s.has_dense and autotune.enabled():
+            autotune.ensure_dense(s)   # split-K tile width for this mesh and device
         if self.Ifdila:
             s.set_dilatancy(self.zp, self.Parr, self.Chyd, self.DilatancyC, self.hs, self.EPermeability, self.P0,
                             self.V0)

@@ -33,6 +33,7 @@
 //      exactly that of the reference's retry loop.
 #include "common.hpp"
 
+#include <algorithm>
 #include <cmath>
 #include <cstdint>
 
@@ -726,6 +727,39 @@ int pq3d_ens_gemm(void* h, int reps, float* ms_per) {
     return 0;
 }
 
+// Re-tile the split-K GEMM with kc source columns per chunk (rounded up to a
+// multiple of 4).  Waits for queued work, drops the captured step graph and
+// reallocates the partial sums; member state is untouched (every step
+// recomputes the GEMM before reading the partials).
+int pq3d_ens_set_kc(void* h, int kc) {
+    clear_error();
+    Ensemble* e = (Ensemble*)h;
+    PQ_CHECK(e, "pq3d_ens_set_kc: null handle");
+    PQ_CHECK(kc >= 4, "pq3d_ens_set_kc: kc must be >= 4 (got %d)", kc);
+    const long long kc4 = std::min<long long>(((long long)kc + 3) / 4 * 4, (e->n + 3) / 4 * 4);
+    const int nchunk = (int)((e->n + kc4 - 1) / kc4);
+    PQ_HIP(hipStreamSynchronize(e->stream));
+    ens_drop_graph(e);
+    float* p = nullptr;
+    const size_t bytes = (size_t)nchunk * 3 * e->B * e->npad * sizeof(float);
+    PQ_HIP(hipMalloc((void**)&p, bytes));
+    PQ_HIP(hipMemset(p, 0, bytes));
+    (void)hipFree(e->partial);
+    e->partial = p;
+    e->kc = (int)kc4;
+    e->nchunk = nchunk;
+    return 0;
+}
+
+int pq3d_ens_get_kc(void* h, int* kc, int* nchunk) {
+    clear_error();
+    Ensemble* e = (Ensemble*)h;
+    PQ_CHECK(e && kc && nchunk, "pq3d_ens_get_kc: null argument");
+    *kc = e->kc;
+    *nchunk = e->nchunk;
+    return 0;
+}
+
 __global__ void ens_sum_partials_kernel(const float* __restrict__ partial, int nchunk, int B, long long n,
                                         long long npad, double* __restrict__ out) {
     const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -751,6 +785,39 @@ int pq3d_ens_gemm_result(void* h, double* host_3Bn) {
     if (err == hipSuccess) err = hipStreamSynchronize(e->stream);
     (void)hipFree(d);
     PQ_HIP(err);
+    return 0;
+}
+
+// Time the product as a global step consumes it: operand preparation, the
+// split-K GEMM and one fixed-order pass over the partial sums (the step's
+// first attempt round reads them once).  ms per product over `reps`.
+int pq3d_ens_time_product(void* h, int reps, float* ms_per) {
+    clear_error();
+    Ensemble* e = (Ensemble*)h;
+    PQ_CHECK(e && ms_per && reps > 0, "pq3d_ens_time_product: bad argument");
+    const long long tot = 3LL * e->B * e->n;
+    double* d = nullptr;
+    PQ_HIP(hipMalloc((void**)&d, tot * sizeof(double)));
+    hipEvent_t a = nullptr, b = nullptr;
+    hipError_t err = hipEventCreate(&a);
+    if (err == hipSuccess) err = hipEventCreate(&b);
+    float ms = 0;
+    if (err == hipSuccess) {
+        for (int r = -1; r < reps; ++r) {  // r = -1 is the untimed warm-up
+            if (r == 0) (void)hipEventRecord(a, e->stream);
+            ens_launch_gemm(e, e->stream);
+            hipLaunchKernelGGL(ens_sum_partials_kernel, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0,
+                               e->stream, e->partial, e->nchunk, e->B, e->n, e->npad, d);
+        }
+        err = hipEventRecord(b, e->stream);
+        if (err == hipSuccess) err = hipEventSynchronize(b);
+        if (err == hipSuccess) err = hipEventElapsedTime(&ms, a, b);
+    }
+    if (a) (void)hipEventDestroy(a);
+    if (b) (void)hipEventDestroy(b);
+    (void)hipFree(d);
+    PQ_HIP(err);
+    *ms_per = ms / reps;
     return 0;
 }
 

@@ -943,18 +943,85 @@ __global__ void reduce_partials_kernel(const double* __restrict__ partial, int n
     dsg[i] = fix_tn ? nl[i] : z + nl[i];
 }
 
+// The traction product as the step consumes it: split-K partials, then their
+// fixed-order sum into the traction-rate fields.
+static void launch_product(DenseSim* s, hipStream_t st) {
+    launch_matvec(s, st);
+    const long long n = s->n;
+    double* F = s->fields;
+    hipLaunchKernelGGL(reduce_partials_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, s->partial,
+                       s->nchunk, n, F + F_NORMAL_LOADING * n, s->fix_tn, F + F_ADOTV1 * n, F + F_ADOTV2 * n,
+                       F + F_DSIGMADT * n);
+}
+
 int pq3d_dense_matvec(void* h) {
     clear_error();
     DenseSim* s = (DenseSim*)h;
     PQ_CHECK(s->mat_type != MAT_NONE || s->hmat, "dense sim: no matrices and no H-matrix attached");
-    launch_matvec(s, 0);
-    const long long n = s->n;
-    double* F = s->fields;
-    hipLaunchKernelGGL(reduce_partials_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, 0, s->partial,
-                       s->nchunk, n, F + F_NORMAL_LOADING * n, s->fix_tn, F + F_ADOTV1 * n, F + F_ADOTV2 * n,
-                       F + F_DSIGMADT * n);
+    launch_product(s, 0);
     PQ_HIP(hipGetLastError());
     PQ_HIP(hipDeviceSynchronize());
+    return 0;
+}
+
+// Re-tile the dense traction product with kc source columns per chunk (the
+// split-K width of matvec_kernel).  Waits for the device, drops the captured
+// step graph and reallocates the partial sums.  Not for a simulation that
+// runs on an attached H-matrix or has no matrices.
+int pq3d_dense_set_kc(void* h, int kc) {
+    clear_error();
+    DenseSim* s = (DenseSim*)h;
+    PQ_CHECK(s, "pq3d_dense_set_kc: null handle");
+    PQ_CHECK(s->mat_type != MAT_NONE && !s->hmat, "pq3d_dense_set_kc: simulation has no dense matrices");
+    PQ_CHECK(kc >= 1, "pq3d_dense_set_kc: kc must be positive (got %d)", kc);
+    const int kcn = (int)std::min<long long>(kc, s->n);
+    const int nchunk = (int)((s->n + kcn - 1) / kcn);
+    PQ_HIP(hipDeviceSynchronize());
+    if (s->graph) {
+        (void)hipGraphExecDestroy(s->graph);
+        s->graph = nullptr;
+    }
+    double* p = nullptr;
+    const size_t bytes = (size_t)nchunk * 3 * s->n * sizeof(double);
+    PQ_HIP(hipMalloc((void**)&p, bytes));
+    PQ_HIP(hipMemset(p, 0, bytes));
+    (void)hipFree(s->partial);
+    s->partial = p;
+    s->kc = kcn;
+    s->nchunk = nchunk;
+    return 0;
+}
+
+int pq3d_dense_get_kc(void* h, int* kc, int* nchunk) {
+    clear_error();
+    DenseSim* s = (DenseSim*)h;
+    PQ_CHECK(s && kc && nchunk, "pq3d_dense_get_kc: null argument");
+    *kc = s->kc;
+    *nchunk = s->nchunk;
+    return 0;
+}
+
+// Time the product with its partial-sum reduction (what a split-K width
+// costs end to end): `reps` launches between two events, ms per product.
+// Writes the same traction rates pq3d_dense_matvec does.
+int pq3d_dense_time_product(void* h, int reps, float* ms_per) {
+    clear_error();
+    DenseSim* s = (DenseSim*)h;
+    PQ_CHECK(s->mat_type != MAT_NONE || s->hmat, "dense sim: no matrices and no H-matrix attached");
+    PQ_CHECK(reps > 0 && ms_per, "pq3d_dense_time_product: reps must be positive");
+    hipEvent_t a, b;
+    PQ_HIP(hipEventCreate(&a));
+    PQ_HIP(hipEventCreate(&b));
+    launch_product(s, 0);
+    PQ_HIP(hipEventRecord(a, 0));
+    for (int r = 0; r < reps; ++r) launch_product(s, 0);
+    PQ_HIP(hipEventRecord(b, 0));
+    PQ_HIP(hipEventSynchronize(b));
+    float ms = 0;
+    PQ_HIP(hipEventElapsedTime(&ms, a, b));
+    *ms_per = ms / reps;
+    (void)hipEventDestroy(a);
+    (void)hipEventDestroy(b);
     return 0;
 }
 

@@ -37,6 +37,9 @@ def _declare(L):
     _lib._sig(L, "pq3d_ens_reset_member", [vp, i, d, d, i])
     _lib._sig(L, "pq3d_ens_gemm", [vp, i, ctypes.POINTER(ctypes.c_float)])
     _lib._sig(L, "pq3d_ens_gemm_result", [vp, dp])
+    _lib._sig(L, "pq3d_ens_time_product", [vp, i, ctypes.POINTER(ctypes.c_float)])
+    _lib._sig(L, "pq3d_ens_set_kc", [vp, i])
+    _lib._sig(L, "pq3d_ens_get_kc", [vp, ctypes.POINTER(i), ctypes.POINTER(i)])
     _lib._sig(L, "pq3d_ens_run", [vp, ll, i, i])
     _lib._sig(L, "pq3d_ens_sync", [vp])
     _lib._sig(L, "pq3d_ens_query", [vp, ctypes.POINTER(ctypes.c_int)])
@@ -109,6 +112,23 @@ class Ensemble:
         f = ctypes.c_float()
         check(_lib.lib().pq3d_ens_gemm(self.h, int(reps), ctypes.byref(f)), "ens gemm")
         return f.value
+
+    def time_product(self, reps=4):
+        """ms per product as a global step uses it (operand preparation, GEMM,
+        one pass over the split-K partial sums)."""
+        f = ctypes.c_float()
+        check(_lib.lib().pq3d_ens_time_product(self.h, int(reps), ctypes.byref(f)), "ens time product")
+        return f.value
+
+    @property
+    def kc(self):
+        """Split-K width of the GEMM tiles (64 rows x 64 members x kc sources)."""
+        kc, nc = ctypes.c_int(), ctypes.c_int()
+        check(_lib.lib().pq3d_ens_get_kc(self.h, ctypes.byref(kc), ctypes.byref(nc)), "ens get kc")
+        return kc.value
+
+    def set_kc(self, kc):
+        check(_lib.lib().pq3d_ens_set_kc(self.h, int(kc)), "ens set kc")
 
     def gemm_result(self):
         out = np.empty((3, self.B, self.n))

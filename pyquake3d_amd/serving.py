@@ -24,7 +24,7 @@ from dataclasses import dataclass, field
 
 import numpy as np
 
-from . import _lib, greenfunc, readmsh
+from . import _lib, autotune, greenfunc, readmsh
 from .QDsim_gpu import DenseSim
 from .ensemble import Ensemble, load_member
 
@@ -75,6 +75,8 @@ class _Mesh:
         self.sim.assemble(*greenfunc.source_vertices(self.nodes, self.tri), xg, ev, self.mu, self.lam,
                           self.halfspace)
         self.ens = Ensemble(self.n, 64, matrices_from=self.sim)
+        if autotune.enabled():
+            autotune.ensure_ensemble(self.ens)   # split-K tile width for this mesh and device
         for m in range(64):
             self.ens.reset_member(m, active=False)
         self.slots = [None] * 64

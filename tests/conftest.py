@@ -16,3 +16,12 @@ def pytest_configure(config):
 def golden_td():
     import numpy as np
     return dict(np.load(os.path.join(ROOT, "tests", "golden", "td_golden.npz"), allow_pickle=False))
+
+
+@pytest.fixture(scope="session", autouse=True)
+def _private_tuning_cache(tmp_path_factory):
+    """Simulations built by the tests tune their split-K widths into a cache of
+    this session's own, not the user's."""
+    if "PQ3D_TUNE_CACHE" not in os.environ:
+        os.environ["PQ3D_TUNE_CACHE"] = str(tmp_path_factory.mktemp("tuning") / "tuning.json")
+    yield
