@@ -363,6 +363,33 @@ def hmatrix_leg(steps=200, warmup=10, graph_steps=8, cpu_budget_s=20.0):
     return res
 
 
+def traction_error(s64, gemm_out, member_u, members=(0, 21, 42, 63)):
+    """Traction rates of the batched path (FP32 Green's functions, MFMA GEMM,
+    split-K partials summed in FP64) against the FP64 dense product of the
+    same FP64-assembled matrices, for the members' slip rates after the timed
+    steps.  Per member and component: ||t - t64||_2 / ||t64||_2 and
+    max|t - t64| / max|t64| (a component that is identically zero, as the
+    normal traction rate of a planar fault, is skipped)."""
+    nl = s64.get("normal_loading")
+    rows = []
+    for m in members:
+        for k in ("slipv1", "slipv2", "rake0"):
+            s64.set(k, member_u[k][m])
+        s64.matvec()
+        ref = [s64.get("AdotV1"), s64.get("AdotV2"), s64.get("dsigmadt") - nl]
+        for q, name in enumerate(("strike", "dip", "normal")):
+            r, g = ref[q], gemm_out[q, m]
+            nr = float(np.linalg.norm(r))
+            if nr == 0.0:
+                continue
+            rows.append({"member": m, "component": name, "rel_l2": float(np.linalg.norm(g - r)) / nr,
+                         "rel_max": float(np.abs(g - r).max() / np.abs(r).max())})
+    return {"members": list(members), "per_result": rows,
+            "max_rel_l2": max(r["rel_l2"] for r in rows), "max_rel_max": max(r["rel_max"] for r in rows),
+            "target_rel": 1e-5,
+            "reference": "FP64 dense product (FP64 matrices, FP64 sums) of the same slip rates"}
+
+
 def _tuned(obj, res, builtin_kc):
     """Autotuner result for the JSON line: the chosen split-K width and its
     time beside the built-in width's (both from the same sweep)."""
@@ -510,6 +537,11 @@ def main():
     ens_info = {"members": MEMBERS, "global_steps": args.steps, "ms_per_global_step": ens_ms / args.steps,
                 "gemm_ms": gemm_ms, "member_sim_time_s_min": float(mem["time"].min()),
                 "member_sim_time_s_max": float(mem["time"].max()), "failed_members": int((mem["status"] != 0).sum())}
+    # traction rates of the members' current slip rates through the batched
+    # path (FP32 storage, MFMA GEMM), checked below against the FP64 product
+    ens.gemm()
+    gemm_out = ens.gemm_result()
+    member_u = {k: ens.get(k) for k in ("slipv1", "slipv2", "rake0")}
     ens.close()
     sim.close()
 
@@ -523,6 +555,10 @@ def main():
     plain64 = k64 / (p64_ms * 1e-3)
     mv64_ms = s64.time_matvec(10)
     log(f"[bench] plain FP64: {plain64:.1f} member-steps/s; matvec {mv64_ms:.3f} ms")
+    terr = traction_error(s64, gemm_out, member_u)
+    del gemm_out, member_u
+    log(f"[bench] batched traction rates vs FP64 product: relative L2 error {terr['max_rel_l2']:.2e}, "
+        f"max-norm {terr['max_rel_max']:.2e} over {len(terr['members'])} members")
     cpu = None
     if not args.no_cpu_baseline:
         mats64 = [s64.download_matrix(k) for k in range(6)]
@@ -582,6 +618,7 @@ def main():
                            "kernel": "matvec_kernel<float> (six FP32 traction matrices, FP64 accumulation)",
                            "alg_bytes_per_launch": mv_alg, "avg_launch_ms": mv_ms},
         "ensemble": ens_info,
+        "traction_error": terr,
         "autotune": tuning,
         "gf_assembly": gf,
         "p99_solve_latency_s": serving["p99_time_to_result_s"] if serving else None,

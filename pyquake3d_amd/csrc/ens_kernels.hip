@@ -103,14 +103,16 @@ __global__ void ens_prep_u_kernel(const double* __restrict__ V1, const double* _
     U2[t] = u2;
 }
 
-// grid: (ceil(n / 64), nchunk); block 64 = one wave owning 64 rows x 64 members.
-// Requires B == 64 and ld % 4 == 0 (rows padded, zero-filled).
-__global__ __launch_bounds__(64) void ens_gemm_kernel(const float* __restrict__ A1s, const float* __restrict__ A2s,
-                                                      const float* __restrict__ Bs, const float* __restrict__ A1d,
-                                                      const float* __restrict__ A2d, const float* __restrict__ Bd,
-                                                      long long ld, long long n, int kc,
-                                                      const float* __restrict__ U1, const float* __restrict__ U2,
-                                                      float* __restrict__ partial, long long npad) {
+// One wave's 64 rows x 64 members: the MFMAs of member subtiles j < L (member
+// subtile j = members j, j + 4, ..., j + 60); subtiles j >= L keep their zero
+// accumulators.
+template <int L>
+__device__ __forceinline__ void ens_gemm_tile(const float* __restrict__ A1s, const float* __restrict__ A2s,
+                                              const float* __restrict__ Bs, const float* __restrict__ A1d,
+                                              const float* __restrict__ A2d, const float* __restrict__ Bd,
+                                              long long ld, long long n, int kc, const float* __restrict__ U1,
+                                              const float* __restrict__ U2, float* __restrict__ partial,
+                                              long long npad) {
     const int lane = threadIdx.x & 63;
     const long long r0 = (long long)blockIdx.x * 64;  // this wave's first row
     const long long k0 = (long long)blockIdx.y * kc;
@@ -157,7 +159,7 @@ __global__ __launch_bounds__(64) void ens_gemm_kernel(const float* __restrict__ 
 #pragma unroll
             for (int i = 0; i < 4; ++i)
 #pragma unroll
-                for (int j = 0; j < 4; ++j) {
+                for (int j = 0; j < L; ++j) {
                     acc[q][i][j] = __builtin_amdgcn_mfma_f32_16x16x4f32(as_[q][i], uu1[j], acc[q][i][j], 0, 0, 0);
                     acc[q][i][j] = __builtin_amdgcn_mfma_f32_16x16x4f32(ad_[q][i], uu2[j], acc[q][i][j], 0, 0, 0);
                 }
@@ -177,6 +179,33 @@ __global__ __launch_bounds__(64) void ens_gemm_kernel(const float* __restrict__ 
                 *reinterpret_cast<float4*>(partial + (((size_t)blockIdx.y * 3 + q) * 64 + member) * npad + prow) = o;
             }
         }
+}
+
+// grid: (ceil(n / 64), nchunk); block 64 = one wave owning 64 rows x 64 members.
+// Requires B == 64 and ld % 4 == 0 (rows padded, zero-filled).  Member
+// subtiles above the highest one holding an active member skip their MFMAs:
+// an inactive member's u is zero, so its results are exactly zero either way
+// (the serving loop fills subtile 0 first, so few requests use few subtiles).
+__global__ __launch_bounds__(64) void ens_gemm_kernel(const float* __restrict__ A1s, const float* __restrict__ A2s,
+                                                      const float* __restrict__ Bs, const float* __restrict__ A1d,
+                                                      const float* __restrict__ A2d, const float* __restrict__ Bd,
+                                                      long long ld, long long n, int kc,
+                                                      const float* __restrict__ U1, const float* __restrict__ U2,
+                                                      float* __restrict__ partial, long long npad,
+                                                      const MemberCtrl* __restrict__ mc) {
+    const unsigned long long act = __ballot(mc[threadIdx.x & 63].active != 0);
+    int L = 0;  // 1 + the highest member subtile with an active member (wave-uniform)
+#pragma unroll
+    for (int j = 0; j < 4; ++j)
+        if (act & (0x1111111111111111ull << j)) L = j + 1;
+    if (L == 4)
+        ens_gemm_tile<4>(A1s, A2s, Bs, A1d, A2d, Bd, ld, n, kc, U1, U2, partial, npad);
+    else if (L == 3)
+        ens_gemm_tile<3>(A1s, A2s, Bs, A1d, A2d, Bd, ld, n, kc, U1, U2, partial, npad);
+    else if (L == 2)
+        ens_gemm_tile<2>(A1s, A2s, Bs, A1d, A2d, Bd, ld, n, kc, U1, U2, partial, npad);
+    else
+        ens_gemm_tile<1>(A1s, A2s, Bs, A1d, A2d, Bd, ld, n, kc, U1, U2, partial, npad);
 }
 
 // ---------------------------------------------------------------------------
@@ -465,7 +494,7 @@ static void ens_launch_gemm(Ensemble* e, hipStream_t st) {
                        f + (size_t)E_SINR0 * items, e->Vpl, e->n, e->B, e->mc, e->ctrl, U1, U2);
     dim3 grid((unsigned)((e->n + 63) / 64), (unsigned)e->nchunk);
     hipLaunchKernelGGL(ens_gemm_kernel, grid, dim3(64), 0, st, e->mats[0], e->mats[1], e->mats[2], e->mats[3],
-                       e->mats[4], e->mats[5], e->ld, e->n, e->kc, U1, U2, e->partial, e->npad);
+                       e->mats[4], e->mats[5], e->ld, e->n, e->kc, U1, U2, e->partial, e->npad, e->mc);
 }
 
 static void ens_launch_step(Ensemble* e, hipStream_t st) {

@@ -89,9 +89,14 @@ class _Mesh:
             self.sim.close()
             self.sim = None
 
+    # slots in member-subtile order (members 0, 4, ..., 60, then 1, 5, ...):
+    # the ensemble GEMM skips the MFMAs of the subtiles above the highest one
+    # holding an active member, so a lightly loaded mesh uses few subtiles
+    SLOT_ORDER = tuple((s % 16) * 4 + s // 16 for s in range(64))
+
     def free_slot(self):
-        for m, r in enumerate(self.slots):
-            if r is None:
+        for m in self.SLOT_ORDER:
+            if self.slots[m] is None:
                 return m
         return None
 

@@ -109,3 +109,27 @@ def test_failed_member_is_masked_and_others_unchanged(mesh):
     assert np.array_equal(ma["time"][keep], mb["time"][keep])
     a.close()
     b.close()
+
+
+def test_gemm_skips_idle_member_subtiles_exactly(mesh):
+    """Member subtile j holds members j, j + 4, ..., j + 60; the GEMM skips the
+    subtiles above the highest one with an active member.  Active members'
+    results must be bit-identical to the all-active product and inactive
+    members' exactly zero, whichever subtiles are live."""
+    sim, mats, cols, xg = mesh
+    e = _ens(mesh)
+    rng = np.random.default_rng(8)
+    n = sim.n
+    e.set("slipv1", rng.lognormal(-18, 3, (B, n)) * rng.choice([-1, 1], (B, n)))
+    e.set("slipv2", rng.lognormal(-18, 3, (B, n)))
+    e.gemm()
+    full = e.gemm_result()
+    for active in ([0, 4, 60], [0, 1], [2], [5, 10], [3, 63], list(range(64))):
+        for m in range(B):
+            e.reset_member(m, active=m in active)
+        e.gemm()
+        got = e.gemm_result()
+        on = np.isin(np.arange(B), active)
+        assert np.array_equal(got[:, on], full[:, on]), active
+        assert not np.any(got[:, ~on]), active
+    e.close()
