@@ -130,6 +130,13 @@ int pq3d_dense_time_matvec(void* sim, int reps, float* ms_per_launch);
 int pq3d_dense_set_kc(void* sim, int kc);
 int pq3d_dense_get_kc(void* sim, int* kc, int* nchunk);
 int pq3d_dense_time_product(void* sim, int reps, float* ms_per_product);
+/* zero-matrix structure (no reference counterpart): scan the six matrices
+ * (mask bit k = matrix k holds only +0/-0) and skip Bs, Bd in the product when
+ * both are zero (planar fault); results are bit-identical.  Rescan after the
+ * matrices change; set_structure(0) streams all six */
+int pq3d_dense_scan_structure(void* sim, int* mask);
+int pq3d_dense_set_structure(void* sim, int mask);
+int pq3d_dense_get_structure(void* sim, int* mask, int* skip_normal);
 /* one accepted step; returns 1 with 'error: dt is too small' in
  * pq3d_last_error() where the reference exits (QDsim_gpu.py:983-985) */
 int pq3d_dense_step(void* sim, double dttry, double* h, double* dtnext, int* nrjct);
@@ -181,6 +188,10 @@ int pq3d_ens_gemm_result(void* ens, double* host_3Bn);
 int pq3d_ens_set_kc(void* ens, int kc);
 int pq3d_ens_get_kc(void* ens, int* kc, int* nchunk);
 int pq3d_ens_time_product(void* ens, int reps, float* ms_per_product);
+/* zero-matrix structure of the GEMM's six matrices, as for the dense product */
+int pq3d_ens_scan_structure(void* ens, int* mask);
+int pq3d_ens_set_structure(void* ens, int mask);
+int pq3d_ens_get_structure(void* ens, int* mask, int* skip_normal);
 /* enqueue nsteps global steps (HIP graph of graph_steps); async != 0 returns
  * without waiting */
 int pq3d_ens_run(void* ens, long long nsteps, int graph_steps, int async);

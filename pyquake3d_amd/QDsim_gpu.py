@@ -58,6 +58,9 @@ def _declare(L):
     _lib._sig(L, "pq3d_dense_time_product", [vp, i, ctypes.POINTER(ctypes.c_float)])
     _lib._sig(L, "pq3d_dense_set_kc", [vp, i])
     _lib._sig(L, "pq3d_dense_get_kc", [vp, ip, ip])
+    _lib._sig(L, "pq3d_dense_scan_structure", [vp, ip])
+    _lib._sig(L, "pq3d_dense_set_structure", [vp, i])
+    _lib._sig(L, "pq3d_dense_get_structure", [vp, ip, ip])
     _lib._sig(L, "pq3d_dense_step", [vp, d, dp, dp, ip])
     _lib._sig(L, "pq3d_dense_run", [vp, ll, i, dp, ip])
     _lib._sig(L, "pq3d_dense_reset_ctrl", [vp])
@@ -105,6 +108,11 @@ class DenseSim:
         self.n = int(n)
         self.mat_type = mat_type
         self.fix_tn = False
+        # matrices that hold only zeros (Bs, Bd of a planar fault) are skipped by
+        # the product; a scan runs before the first product after any change
+        self.exploit_structure = True
+        self.mat_gen = 0
+        self._structure_gen = None
         ptrs = (ctypes.c_void_p * 6)()
         ld = ctypes.c_longlong()
         check(L.pq3d_dense_matrix_ptrs(h, ptrs, ctypes.byref(ld)), "matrix_ptrs")
@@ -181,6 +189,7 @@ class DenseSim:
             col[:, :n] = ((bits + rnd) >> 16).astype(np.uint16)
         check(_lib.lib().pq3d_memcpy_h2d(ctypes.c_void_p(self.mat_ptrs[k]), col.ctypes.data_as(ctypes.c_void_p),
                                          n * self.ld * esz), "upload matrix")
+        self.matrices_changed()
 
     def download_matrix(self, k):
         """Device matrix -> host float64 (receiver x source)."""
@@ -198,17 +207,44 @@ class DenseSim:
             raise ValueError("assemble into FP64 or FP32 storage")
         greenfunc.assemble_device(P1, P2, P3, xg, ev, mu, lam, halfspace, self.mat_ptrs, self.ld,
                                   self.mat_type)
+        self.matrices_changed()
+
+    def matrices_changed(self):
+        """Mark the matrices as rewritten (call after writing through
+        ``mat_ptrs`` directly); the next product rescans their structure."""
+        self.mat_gen += 1
+
+    def _sync_structure(self):
+        want = self.mat_gen if (self.exploit_structure and self.has_dense) else -1
+        if want == self._structure_gen:
+            return
+        if want == -1:
+            check(_lib.lib().pq3d_dense_set_structure(self.h, 0), "set_structure")
+        else:
+            check(_lib.lib().pq3d_dense_scan_structure(self.h, None), "scan_structure")
+        self._structure_gen = want
+
+    @property
+    def structure(self):
+        """(zero-matrix mask, whether the product skips Bs and Bd)."""
+        self._sync_structure()
+        m, sk = ctypes.c_int(), ctypes.c_int()
+        check(_lib.lib().pq3d_dense_get_structure(self.h, ctypes.byref(m), ctypes.byref(sk)), "get_structure")
+        return m.value, bool(sk.value)
 
     def matvec(self):
+        self._sync_structure()
         check(_lib.lib().pq3d_dense_matvec(self.h), "matvec")
 
     def time_matvec(self, reps=20):
+        self._sync_structure()
         f = ctypes.c_float()
         check(_lib.lib().pq3d_dense_time_matvec(self.h, int(reps), ctypes.byref(f)), "time_matvec")
         return f.value
 
     def time_product(self, reps=5):
         """ms per traction product including the split-K partial-sum pass."""
+        self._sync_structure()
         f = ctypes.c_float()
         check(_lib.lib().pq3d_dense_time_product(self.h, int(reps), ctypes.byref(f)), "time_product")
         return f.value
@@ -224,6 +260,7 @@ class DenseSim:
         check(_lib.lib().pq3d_dense_set_kc(self.h, int(kc)), "set_kc")
 
     def step(self, dttry):
+        self._sync_structure()
         h = ctypes.c_double()
         dtn = ctypes.c_double()
         nr = ctypes.c_int()
@@ -234,6 +271,7 @@ class DenseSim:
         return h.value, dtn.value, nr.value
 
     def run(self, nsteps, graph_steps=8, records=False):
+        self._sync_structure()
         st = ctypes.c_int()
         rec = np.zeros((nsteps, 4)) if records else None
         check(_lib.lib().pq3d_dense_run(self.h, int(nsteps), int(graph_steps),

@@ -75,12 +75,19 @@ def _device_key():
     return f"{info['name']}|cu{info['cu']}"
 
 
+def _nq(obj):
+    """Traction components the product computes (2 when Bs and Bd are skipped)."""
+    st = getattr(obj, "structure", None)
+    return 2 if (st is not None and st[1]) else 3
+
+
 def ensemble_key(ens):
-    return f"{_device_key()}|ens_gemm|n{ens.n}|B{ens.B}"
+    return f"{_device_key()}|ens_gemm|n{ens.n}|B{ens.B}|nq{_nq(ens)}"
 
 
 def dense_key(sim):
-    return f"{_device_key()}|matvec_{_MAT_NAMES[sim.mat_type]}|n{sim.n}|fixtn{int(bool(sim.fix_tn))}"
+    return (f"{_device_key()}|matvec_{_MAT_NAMES[sim.mat_type]}|n{sim.n}|fixtn{int(bool(sim.fix_tn))}"
+            f"|nq{_nq(sim)}")
 
 
 def ensemble_candidates(n):

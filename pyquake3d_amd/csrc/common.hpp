@@ -47,4 +47,9 @@ int scratch_get(Scratch& s, size_t bytes, void** out);
 enum MatType : int { MAT_F64 = 0, MAT_F32 = 1, MAT_BF16 = 2, MAT_NONE = 3 };
 inline size_t mat_elem_bytes(int t) { return t == MAT_F64 ? 8 : (t == MAT_F32 ? 4 : (t == MAT_BF16 ? 2 : 0)); }
 
+// Bit k of *mask set iff device matrix k (ld x n elements of type mat_type;
+// ld * elem bytes a multiple of 16) holds only +0/-0.  Reads every matrix
+// once; synchronous.
+int scan_zero_matrices(void* const* mats, int nmat, int mat_type, long long ld, long long n, int* mask);
+
 }  // namespace pq3d

@@ -105,8 +105,9 @@ __global__ void ens_prep_u_kernel(const double* __restrict__ V1, const double* _
 
 // One wave's 64 rows x 64 members: the MFMAs of member subtiles j < L (member
 // subtile j = members j, j + 4, ..., j + 60); subtiles j >= L keep their zero
-// accumulators.
-template <int L>
+// accumulators.  NQ = 2 skips the normal component (Bs = Bd = 0): its loads
+// and MFMAs, and its partial sums are written as the zeros they would be.
+template <int L, int NQ>
 __device__ __forceinline__ void ens_gemm_tile(const float* __restrict__ A1s, const float* __restrict__ A2s,
                                               const float* __restrict__ Bs, const float* __restrict__ A1d,
                                               const float* __restrict__ A2d, const float* __restrict__ Bd,
@@ -133,14 +134,16 @@ __device__ __forceinline__ void ens_gemm_tile(const float* __restrict__ A1s, con
 #pragma unroll
             for (int j = 0; j < 4; ++j) acc[q][i][j] = v4f{0.f, 0.f, 0.f, 0.f};
 
-    float4 a[6], u1, u2;
+    float4 a[6] = {}, u1, u2;
     bool ok = false;  // the loaded k is inside the chunk (else u counts as 0)
     auto load = [&](long long k) {
         const long long s = k + lk;
         ok = s < k1;
         const long long sc = ok ? s : k1 - 1;
 #pragma unroll
-        for (int m = 0; m < 6; ++m) a[m] = *reinterpret_cast<const float4*>(mats[m] + (size_t)sc * ld + row);
+        for (int m = 0; m < 6; ++m)
+            if (NQ == 3 || (m != 2 && m != 5))
+                a[m] = *reinterpret_cast<const float4*>(mats[m] + (size_t)sc * ld + row);
         u1 = *reinterpret_cast<const float4*>(U1 + (size_t)sc * 64 + 4 * lr);
         u2 = *reinterpret_cast<const float4*>(U2 + (size_t)sc * 64 + 4 * lr);
     };
@@ -155,7 +158,7 @@ __device__ __forceinline__ void ens_gemm_tile(const float* __restrict__ A1s, con
         const float uu2[4] = {ok ? u2.x : 0.f, ok ? u2.y : 0.f, ok ? u2.z : 0.f, ok ? u2.w : 0.f};
         if (k + 4 < k1) load(k + 4);
 #pragma unroll
-        for (int q = 0; q < 3; ++q) {
+        for (int q = 0; q < NQ; ++q) {
 #pragma unroll
             for (int i = 0; i < 4; ++i)
 #pragma unroll
@@ -176,6 +179,7 @@ __device__ __forceinline__ void ens_gemm_tile(const float* __restrict__ A1s, con
             for (int v = 0; v < 4; ++v) {
                 const long long prow = r0 + 4 * (4 * lk + v);
                 float4 o = make_float4(acc[q][0][j][v], acc[q][1][j][v], acc[q][2][j][v], acc[q][3][j][v]);
+                if (q >= NQ) o = make_float4(0.f, 0.f, 0.f, 0.f);
                 *reinterpret_cast<float4*>(partial + (((size_t)blockIdx.y * 3 + q) * 64 + member) * npad + prow) = o;
             }
         }
@@ -186,6 +190,7 @@ __device__ __forceinline__ void ens_gemm_tile(const float* __restrict__ A1s, con
 // subtiles above the highest one holding an active member skip their MFMAs:
 // an inactive member's u is zero, so its results are exactly zero either way
 // (the serving loop fills subtile 0 first, so few requests use few subtiles).
+template <int NQ>
 __global__ __launch_bounds__(64) void ens_gemm_kernel(const float* __restrict__ A1s, const float* __restrict__ A2s,
                                                       const float* __restrict__ Bs, const float* __restrict__ A1d,
                                                       const float* __restrict__ A2d, const float* __restrict__ Bd,
@@ -199,13 +204,13 @@ __global__ __launch_bounds__(64) void ens_gemm_kernel(const float* __restrict__ 
     for (int j = 0; j < 4; ++j)
         if (act & (0x1111111111111111ull << j)) L = j + 1;
     if (L == 4)
-        ens_gemm_tile<4>(A1s, A2s, Bs, A1d, A2d, Bd, ld, n, kc, U1, U2, partial, npad);
+        ens_gemm_tile<4, NQ>(A1s, A2s, Bs, A1d, A2d, Bd, ld, n, kc, U1, U2, partial, npad);
     else if (L == 3)
-        ens_gemm_tile<3>(A1s, A2s, Bs, A1d, A2d, Bd, ld, n, kc, U1, U2, partial, npad);
+        ens_gemm_tile<3, NQ>(A1s, A2s, Bs, A1d, A2d, Bd, ld, n, kc, U1, U2, partial, npad);
     else if (L == 2)
-        ens_gemm_tile<2>(A1s, A2s, Bs, A1d, A2d, Bd, ld, n, kc, U1, U2, partial, npad);
+        ens_gemm_tile<2, NQ>(A1s, A2s, Bs, A1d, A2d, Bd, ld, n, kc, U1, U2, partial, npad);
     else
-        ens_gemm_tile<1>(A1s, A2s, Bs, A1d, A2d, Bd, ld, n, kc, U1, U2, partial, npad);
+        ens_gemm_tile<1, NQ>(A1s, A2s, Bs, A1d, A2d, Bd, ld, n, kc, U1, U2, partial, npad);
 }
 
 // ---------------------------------------------------------------------------
@@ -465,6 +470,8 @@ struct Ensemble {
     hipStream_t stream = nullptr;
     hipGraphExec_t graph = nullptr;
     int graph_steps = 0;
+    int zero_mask = 0;    // bit k: matrix k holds only zeros (from the last scan)
+    int skip_normal = 0;  // Bs = Bd = 0: the GEMM skips the normal component
 };
 
 static EnsArgs ens_args(Ensemble* e) {
@@ -493,8 +500,12 @@ static void ens_launch_gemm(Ensemble* e, hipStream_t st) {
                        f + (size_t)E_SLIPV1 * items, f + (size_t)E_SLIPV2 * items, f + (size_t)E_COSR0 * items,
                        f + (size_t)E_SINR0 * items, e->Vpl, e->n, e->B, e->mc, e->ctrl, U1, U2);
     dim3 grid((unsigned)((e->n + 63) / 64), (unsigned)e->nchunk);
-    hipLaunchKernelGGL(ens_gemm_kernel, grid, dim3(64), 0, st, e->mats[0], e->mats[1], e->mats[2], e->mats[3],
-                       e->mats[4], e->mats[5], e->ld, e->n, e->kc, U1, U2, e->partial, e->npad, e->mc);
+    if (e->skip_normal)
+        hipLaunchKernelGGL(ens_gemm_kernel<2>, grid, dim3(64), 0, st, e->mats[0], e->mats[1], e->mats[2], e->mats[3],
+                           e->mats[4], e->mats[5], e->ld, e->n, e->kc, U1, U2, e->partial, e->npad, e->mc);
+    else
+        hipLaunchKernelGGL(ens_gemm_kernel<3>, grid, dim3(64), 0, st, e->mats[0], e->mats[1], e->mats[2], e->mats[3],
+                           e->mats[4], e->mats[5], e->ld, e->n, e->kc, U1, U2, e->partial, e->npad, e->mc);
 }
 
 static void ens_launch_step(Ensemble* e, hipStream_t st) {
@@ -541,7 +552,7 @@ int pq3d_ens_create(void** out, long long n, int B, void* const* d_mats, long lo
         PQ_HIP(hipGetDevice(&dev));
         hipDeviceProp_t p;
         PQ_HIP(hipGetDeviceProperties(&p, dev));
-        PQ_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, ens_gemm_kernel, 64, 0));
+        PQ_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, ens_gemm_kernel<3>, 64, 0));
         const double slots = (double)per_cu * p.multiProcessorCount;
         const long long tiles = (n + 63) / 64;
         double best = -1.0;
@@ -777,6 +788,52 @@ int pq3d_ens_set_kc(void* h, int kc) {
     e->partial = p;
     e->kc = (int)kc4;
     e->nchunk = nchunk;
+    return 0;
+}
+
+static void ens_set_structure(Ensemble* e, int mask) {
+    const int skip = (mask & 0x24) == 0x24;
+    if (skip != e->skip_normal) ens_drop_graph(e);
+    e->zero_mask = mask;
+    e->skip_normal = skip;
+}
+
+// Scan the six matrices (owned or borrowed; reads each once) and let the GEMM
+// skip the normal component when Bs and Bd hold only zeros (a planar fault).
+// Results are bit-identical to the full GEMM.  Call again after the matrices
+// change.  mask_out (optional): bit k set iff matrix k is all zero.
+int pq3d_ens_scan_structure(void* h, int* mask_out) {
+    clear_error();
+    Ensemble* e = (Ensemble*)h;
+    PQ_CHECK(e, "pq3d_ens_scan_structure: null handle");
+    PQ_HIP(hipStreamSynchronize(e->stream));
+    void* m[6];
+    for (int k = 0; k < 6; ++k) m[k] = (void*)e->mats[k];
+    int mask = 0;
+    int rc = scan_zero_matrices(m, 6, MAT_F32, e->ld, e->n, &mask);
+    if (rc) return rc;
+    ens_set_structure(e, mask);
+    if (mask_out) *mask_out = mask;
+    return 0;
+}
+
+// Override the structure (0 = the full GEMM); the caller vouches that every
+// matrix named in mask holds only zeros.
+int pq3d_ens_set_structure(void* h, int mask) {
+    clear_error();
+    Ensemble* e = (Ensemble*)h;
+    PQ_CHECK(e && mask >= 0 && mask < 64, "pq3d_ens_set_structure: bad argument");
+    PQ_HIP(hipStreamSynchronize(e->stream));
+    ens_set_structure(e, mask);
+    return 0;
+}
+
+int pq3d_ens_get_structure(void* h, int* mask, int* skip_normal) {
+    clear_error();
+    Ensemble* e = (Ensemble*)h;
+    PQ_CHECK(e && mask && skip_normal, "pq3d_ens_get_structure: null argument");
+    *mask = e->zero_mask;
+    *skip_normal = e->skip_normal;
     return 0;
 }
 

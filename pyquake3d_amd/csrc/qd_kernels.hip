@@ -587,6 +587,8 @@ struct DenseSim {
     int nblocks = 0;
     double V0 = 1e-6, Vpl = 1e-9, eta = 0.0;
     int fix_tn = 0, err_mode = 1;
+    int zero_mask = 0;             // bit k: matrix k holds only zeros (from the last scan)
+    int skip_normal = 0;           // Bs and Bd are zero: the product skips them
     long long spin_limit = 20000000;
     hipGraphExec_t graph = nullptr;
     int graph_steps = 0;
@@ -629,7 +631,8 @@ static void launch_matvec_t(DenseSim* s, hipStream_t st) {
     dim3 grid((unsigned)((s->n + 256 * VN - 1) / (256 * VN)), (unsigned)s->nchunk);
     double* F = s->fields;
     const long long n = s->n;
-    if (s->fix_tn)
+    // with Bs = Bd = 0 the normal partials are exactly +0 either way
+    if (s->fix_tn || s->skip_normal)
         hipLaunchKernelGGL((matvec_kernel<T, true>), grid, dim3(256), 0, st, M, s->ld, n, s->kc, F + F_SLIPV1 * n,
                            F + F_SLIPV2 * n, F + F_COSR0 * n, F + F_SINR0 * n, s->Vpl, s->partial);
     else
@@ -941,6 +944,56 @@ __global__ void reduce_partials_kernel(const double* __restrict__ partial, int n
     a1[i] = x;
     a2[i] = y;
     dsg[i] = fix_tn ? nl[i] : z + nl[i];
+}
+
+// The captured step graph holds the matvec variant, so a change drops it.
+static void set_structure(DenseSim* s, int mask) {
+    const int skip = (mask & 0x24) == 0x24;
+    if (skip != s->skip_normal && s->graph) {
+        (void)hipGraphExecDestroy(s->graph);
+        s->graph = nullptr;
+    }
+    s->zero_mask = mask;
+    s->skip_normal = skip;
+}
+
+// Structure of the dense matrices: scan them (reads all six once) and let the
+// product skip Bs and Bd when both hold only zeros (a planar fault: the
+// normal traction of in-plane slip vanishes by mirror symmetry).  The results
+// are bit-identical to the full product.  mask_out (optional): bit k set iff
+// matrix k is all zero.
+int pq3d_dense_scan_structure(void* h, int* mask_out) {
+    clear_error();
+    DenseSim* s = (DenseSim*)h;
+    PQ_CHECK(s, "pq3d_dense_scan_structure: null handle");
+    int mask = 0;
+    if (s->mat_type != MAT_NONE) {
+        PQ_HIP(hipDeviceSynchronize());
+        int rc = scan_zero_matrices(s->mats, 6, s->mat_type, s->ld, s->n, &mask);
+        if (rc) return rc;
+    }
+    set_structure(s, mask);
+    if (mask_out) *mask_out = mask;
+    return 0;
+}
+
+// Override the structure (mask as from the scan; 0 = stream all six matrices).
+// The caller vouches that every matrix named in mask holds only zeros.
+int pq3d_dense_set_structure(void* h, int mask) {
+    clear_error();
+    DenseSim* s = (DenseSim*)h;
+    PQ_CHECK(s && mask >= 0 && mask < 64, "pq3d_dense_set_structure: bad argument");
+    set_structure(s, mask);
+    return 0;
+}
+
+int pq3d_dense_get_structure(void* h, int* mask, int* skip_normal) {
+    clear_error();
+    DenseSim* s = (DenseSim*)h;
+    PQ_CHECK(s && mask && skip_normal, "pq3d_dense_get_structure: null argument");
+    *mask = s->zero_mask;
+    *skip_normal = s->skip_normal;
+    return 0;
 }
 
 // The traction product as the step consumes it: split-K partials, then their

@@ -41,6 +41,47 @@ int scratch_get(Scratch& s, size_t bytes, void** out) {
     return 0;
 }
 
+// flag[0] = 1 if any 16-byte word of p[0, nwords) has a bit set outside the
+// sign bits described by `keep` (the per-32-bit-lane mask of magnitude bits).
+__global__ void nonzero_words_kernel(const uint4* __restrict__ p, long long nwords, uint4 keep, int* flag) {
+    bool nz = false;
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < nwords;
+         i += (long long)gridDim.x * blockDim.x) {
+        const uint4 w = p[i];
+        nz |= ((w.x & keep.x) | (w.y & keep.y) | (w.z & keep.z) | (w.w & keep.w)) != 0u;
+    }
+    if (__any(nz) && (threadIdx.x & 63) == 0) flag[0] = 1;  // every writer stores 1
+}
+
+int scan_zero_matrices(void* const* mats, int nmat, int mat_type, long long ld, long long n, int* mask) {
+    PQ_CHECK(mat_type == MAT_F64 || mat_type == MAT_F32 || mat_type == MAT_BF16, "scan: bad matrix type %d",
+             mat_type);
+    const size_t bytes = (size_t)ld * (size_t)n * mat_elem_bytes(mat_type);
+    PQ_CHECK(bytes % 16 == 0, "scan: matrix bytes must be a multiple of 16");
+    uint4 keep;
+    if (mat_type == MAT_F32) keep = make_uint4(0x7fffffffu, 0x7fffffffu, 0x7fffffffu, 0x7fffffffu);
+    else if (mat_type == MAT_BF16) keep = make_uint4(0x7fff7fffu, 0x7fff7fffu, 0x7fff7fffu, 0x7fff7fffu);
+    else keep = make_uint4(0xffffffffu, 0x7fffffffu, 0xffffffffu, 0x7fffffffu);  // low word, high word
+    int* d = nullptr;
+    PQ_HIP(hipMalloc((void**)&d, 8 * sizeof(int)));
+    hipError_t e = hipMemset(d, 0, 8 * sizeof(int));
+    const long long nwords = (long long)(bytes / 16);
+    for (int k = 0; k < nmat && e == hipSuccess; ++k) {
+        if (!mats[k]) continue;
+        hipLaunchKernelGGL(nonzero_words_kernel, dim3(2048), dim3(256), 0, 0, (const uint4*)mats[k], nwords, keep,
+                           d + k);
+        e = hipGetLastError();
+    }
+    int h[8] = {0};
+    if (e == hipSuccess) e = hipMemcpy(h, d, 8 * sizeof(int), hipMemcpyDeviceToHost);
+    (void)hipFree(d);
+    PQ_HIP(e);
+    *mask = 0;
+    for (int k = 0; k < nmat; ++k)
+        if (mats[k] && !h[k]) *mask |= 1 << k;
+    return 0;
+}
+
 }  // namespace pq3d
 
 using namespace pq3d;

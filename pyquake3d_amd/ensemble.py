@@ -40,6 +40,9 @@ def _declare(L):
     _lib._sig(L, "pq3d_ens_time_product", [vp, i, ctypes.POINTER(ctypes.c_float)])
     _lib._sig(L, "pq3d_ens_set_kc", [vp, i])
     _lib._sig(L, "pq3d_ens_get_kc", [vp, ctypes.POINTER(i), ctypes.POINTER(i)])
+    _lib._sig(L, "pq3d_ens_scan_structure", [vp, ctypes.POINTER(i)])
+    _lib._sig(L, "pq3d_ens_set_structure", [vp, i])
+    _lib._sig(L, "pq3d_ens_get_structure", [vp, ctypes.POINTER(i), ctypes.POINTER(i)])
     _lib._sig(L, "pq3d_ens_run", [vp, ll, i, i])
     _lib._sig(L, "pq3d_ens_sync", [vp])
     _lib._sig(L, "pq3d_ens_query", [vp, ctypes.POINTER(ctypes.c_int)])
@@ -68,6 +71,34 @@ class Ensemble:
         check(L.pq3d_ens_matrix_ptrs(h, ptrs, ctypes.byref(ld)), "matrix_ptrs")
         self.mat_ptrs = [int(p) if p else 0 for p in ptrs]
         self.ld = ld.value
+        # the GEMM skips Bs and Bd when both hold only zeros (planar fault);
+        # rescanned before the first product after the matrices change
+        self.exploit_structure = True
+        self.mat_gen = 0
+        self._structure_gen = None
+
+    def matrices_changed(self):
+        """Mark owned matrices as rewritten through ``mat_ptrs``."""
+        self.mat_gen += 1
+
+    def _sync_structure(self):
+        src = self.__dict__.get("_keep")
+        gen = (src.mat_gen if src is not None else self.mat_gen) if self.exploit_structure else -1
+        if gen == self._structure_gen:
+            return
+        if gen == -1:
+            check(_lib.lib().pq3d_ens_set_structure(self.h, 0), "ens set structure")
+        else:
+            check(_lib.lib().pq3d_ens_scan_structure(self.h, None), "ens scan structure")
+        self._structure_gen = gen
+
+    @property
+    def structure(self):
+        """(zero-matrix mask, whether the GEMM skips Bs and Bd)."""
+        self._sync_structure()
+        m, sk = ctypes.c_int(), ctypes.c_int()
+        check(_lib.lib().pq3d_ens_get_structure(self.h, ctypes.byref(m), ctypes.byref(sk)), "ens get structure")
+        return m.value, bool(sk.value)
 
     # ---- fields
     def set(self, name, arr):
@@ -109,6 +140,7 @@ class Ensemble:
 
     # ---- execution
     def gemm(self, reps=0):
+        self._sync_structure()
         f = ctypes.c_float()
         check(_lib.lib().pq3d_ens_gemm(self.h, int(reps), ctypes.byref(f)), "ens gemm")
         return f.value
@@ -116,6 +148,7 @@ class Ensemble:
     def time_product(self, reps=4):
         """ms per product as a global step uses it (operand preparation, GEMM,
         one pass over the split-K partial sums)."""
+        self._sync_structure()
         f = ctypes.c_float()
         check(_lib.lib().pq3d_ens_time_product(self.h, int(reps), ctypes.byref(f)), "ens time product")
         return f.value
@@ -136,6 +169,7 @@ class Ensemble:
         return out
 
     def run(self, nsteps, graph_steps=4, wait=True):
+        self._sync_structure()
         check(_lib.lib().pq3d_ens_run(self.h, int(nsteps), int(graph_steps), 0 if wait else 1), "ens run")
 
     def sync(self):

@@ -127,3 +127,60 @@ def test_stepping_after_retile(mesh, tmp_path, monkeypatch):
     assert np.abs(sa - sb).max() <= 1e-10 * np.abs(sa).max()
     a.close()
     b.close()
+
+
+def test_zero_normal_matrices_skipped_bit_identically(mesh):
+    """Planar vertical fault: Bs = Bd = 0 exactly (mirror symmetry).  The scan
+    finds them, the dense product and the ensemble GEMM skip them, and every
+    result is bit-identical to streaming all six matrices."""
+    from pyquake3d_amd import _lib, synth
+    from pyquake3d_amd.ensemble import Ensemble, init_members
+    a = _sim(mesh, _lib.MAT_F32)
+    b = _sim(mesh, _lib.MAT_F32)
+    assert not np.any(a.download_matrix(2)) and not np.any(a.download_matrix(5))
+    assert a.structure == (0b100100, True)
+    b.exploit_structure = False
+    assert b.structure == (0, False)
+    ra, rb = [], []
+    for s, rec in ((a, ra), (b, rb)):
+        dt = 1e-3
+        for _ in range(20):
+            h, dt, _ = s.step(dt)
+            rec.append((h, dt))
+        st, r = s.run(16, graph_steps=8, records=True)
+        assert st == 0
+        rec.extend(map(tuple, r[:, :2]))
+    assert np.array_equal(np.array(ra), np.array(rb))
+    for name in ("state", "Tt1o", "Tt2o", "Tno", "slipv", "dsigmadt"):
+        assert np.array_equal(a.get(name), b.get(name)), name
+    xg = mesh[3]
+    cols = np.stack([synth.member_params(xg, m) for m in range(B)])
+    e1 = Ensemble(a.n, B, matrices_from=a)
+    e2 = Ensemble(a.n, B, matrices_from=a)
+    e2.exploit_structure = False
+    for e in (e1, e2):
+        init_members(e, cols, V0, VPL, MU, CS)
+    assert e1.structure[1] and not e2.structure[1]
+    e1.gemm()
+    e2.gemm()
+    assert np.array_equal(e1.gemm_result(), e2.gemm_result())
+    e1.run(10, graph_steps=5)
+    e2.run(10, graph_steps=5)
+    for name in ("state", "Tt1o", "slip", "dsigmadt"):
+        assert np.array_equal(e1.get(name), e2.get(name)), name
+    for o in (e1, e2, a, b):
+        o.close()
+
+
+def test_rough_fault_streams_all_matrices():
+    from pyquake3d_amd import _lib, greenfunc, readmsh, synth
+    from pyquake3d_amd.QDsim_gpu import DenseSim
+    nodes_km, tri = synth.rough_fault_mesh(12, 6, length_km=24.0, depth_km=12.0)
+    nodes = nodes_km * 1e3
+    ev, xg = readmsh.get_eleVec(nodes, tri, False)
+    s = DenseSim(len(tri), _lib.MAT_F32)
+    s.assemble(*greenfunc.source_vertices(nodes, tri), xg, ev, MU, LAM, True)
+    assert s.structure == (0, False)
+    s.upload_matrix(2, np.zeros((s.n, s.n)))   # one zero matrix is not enough
+    assert s.structure == (0b100, False)
+    s.close()
