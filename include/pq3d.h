@@ -180,6 +180,9 @@ int pq3d_ens_set_params(void* ens, double V0, double Vpl, double eta);
 int pq3d_ens_members(void* ens, double* d6, long long* i6);
 /* (re)start a slot: time 0, trial step h0, finish when time >= t_end (0: never) */
 int pq3d_ens_reset_member(void* ens, int member, double h0, double t_end, int active);
+/* one operand preparation + GEMM of the members' current slip rates, then
+ * `reps` further launches of the GEMM kernel alone between two events (ms per
+ * GEMM launch; reps = 0 only computes) */
 int pq3d_ens_gemm(void* ens, int reps, float* ms_per_launch);
 int pq3d_ens_gemm_result(void* ens, double* host_3Bn);
 /* split-K tile width of the GEMM (64 receivers x 64 members x kc sources per
