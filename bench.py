@@ -398,10 +398,10 @@ def _tuned(obj, res, builtin_kc):
     time beside the built-in width's (both from the same sweep)."""
     table = dict(res["table"])
     kc_b = min(table, key=lambda k: abs(k - builtin_kc))
-    log(f"[bench] autotune {res['key'].split('|')[2]}: kc {res['kc']} ({res['ms']:.3f} ms per product), "
-        f"built-in kc {builtin_kc}: {table[kc_b]:.3f} ms, {len(table)} candidates")
-    return {"kc": res["kc"], "ms_per_product": res["ms"], "builtin_kc": builtin_kc,
-            "builtin_ms_per_product": table[kc_b], "candidates": len(table),
+    log(f"[bench] autotune {res['key'].split('|')[2]}: kc {res['kc']} ({table[res['kc']]:.3f} ms per product in "
+        f"the sweep), built-in kc {builtin_kc}: {table[kc_b]:.3f} ms, {len(table)} candidates")
+    return {"kc": res["kc"], "ms_per_product": table[res["kc"]], "final_median_ms": res["ms"],
+            "builtin_kc": builtin_kc, "builtin_ms_per_product": table[kc_b], "candidates": len(table),
             "table": [[k, round(v, 5)] for k, v in res["table"]]}
 
 

@@ -105,15 +105,25 @@ def dense_candidates(n):
     return [kc for kc in _DENSE_KC if kc < n] + [n] if n > 0 else []
 
 
-def _sweep(obj, candidates, reps, trials):
+def _sweep(obj, candidates, reps, trials, finalists=4, final_trials=5):
+    """Time every candidate (best of `trials`), then re-time the `finalists`
+    fastest `final_trials` times each, interleaved so that clock and thermal
+    drift hit them alike, and keep the lowest median."""
     table = []
     for kc in candidates:
         obj.set_kc(kc)
         ms = min(obj.time_product(reps) for _ in range(trials))
         table.append((int(obj.kc), float(ms)))
-    best = min(table, key=lambda t: t[1])
-    obj.set_kc(best[0])
-    return best, table
+    top = [kc for kc, _ in sorted(table, key=lambda t: t[1])[:finalists]]
+    runs = {kc: [] for kc in top}
+    for _ in range(final_trials):
+        for kc in top:
+            obj.set_kc(kc)
+            runs[kc].append(obj.time_product(reps))
+    med = {kc: sorted(v)[len(v) // 2] for kc, v in runs.items()}
+    kc = min(med, key=med.get)
+    obj.set_kc(kc)
+    return (kc, float(med[kc])), table
 
 
 def tune_ensemble(ens, candidates=None, reps=4, trials=2, use_cache=True):

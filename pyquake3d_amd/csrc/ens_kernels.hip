@@ -82,25 +82,45 @@ struct EnsArgs {
 };
 
 // ---------------------------------------------------------------------------
-__global__ void ens_prep_u_kernel(const double* __restrict__ V1, const double* __restrict__ V2,
-                                  const double* __restrict__ c0, const double* __restrict__ s0, double Vpl,
-                                  long long n, int B, MemberCtrl* __restrict__ mc, EnsCtrl* __restrict__ ctrl,
-                                  float* __restrict__ U1, float* __restrict__ U2) {
-    // thread per (source j, member m); writes U[j*B + m]
-    const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t < B) mc[t].accepted = 0;
-    if (t == 0) ctrl->gsteps += 1;
-    if (t >= n * B) return;
-    const long long j = t / B;
-    const int m = (int)(t - j * B);
-    const size_t idx = (size_t)m * n + j;
-    float u1 = 0.f, u2 = 0.f;
-    if (mc[m].active) {
-        u1 = (float)__dsub_rn(V1[idx], __dmul_rn(Vpl, c0[idx]));
-        u2 = (float)__dsub_rn(V2[idx], __dmul_rn(Vpl, s0[idx]));
+// u = V - Vpl (cos rake0, sin rake0) of every active member (zero for the
+// others) as FP32 U[source][member].  Workgroup = 64 sources x 64 members:
+// member-major rows are read along the sources and written along the members
+// through an LDS transpose, so both sides are coalesced.  Requires B == 64.
+__global__ __launch_bounds__(256) void ens_prep_u_kernel(const double* __restrict__ V1,
+                                                         const double* __restrict__ V2,
+                                                         const double* __restrict__ c0,
+                                                         const double* __restrict__ s0, double Vpl, long long n,
+                                                         int B, MemberCtrl* __restrict__ mc,
+                                                         EnsCtrl* __restrict__ ctrl, float* __restrict__ U1,
+                                                         float* __restrict__ U2) {
+    __shared__ float t1[64][65], t2[64][65];  // [member][source], padded against bank conflicts
+    const int tid = threadIdx.x;
+    if (blockIdx.x == 0) {
+        if (tid < B) mc[tid].accepted = 0;
+        if (tid == 0) ctrl->gsteps += 1;
     }
-    U1[t] = u1;
-    U2[t] = u2;
+    const long long j0 = (long long)blockIdx.x * 64;
+    for (int it = tid; it < 64 * 64; it += 256) {
+        const int m = it >> 6, jj = it & 63;
+        const long long j = j0 + jj;
+        float u1 = 0.f, u2 = 0.f;
+        if (j < n && mc[m].active) {
+            const size_t idx = (size_t)m * n + j;
+            u1 = (float)__dsub_rn(V1[idx], __dmul_rn(Vpl, c0[idx]));
+            u2 = (float)__dsub_rn(V2[idx], __dmul_rn(Vpl, s0[idx]));
+        }
+        t1[m][jj] = u1;
+        t2[m][jj] = u2;
+    }
+    __syncthreads();
+    for (int it = tid; it < 64 * 64; it += 256) {
+        const int jj = it >> 6, m = it & 63;
+        const long long j = j0 + jj;
+        if (j < n) {
+            U1[(size_t)j * 64 + m] = t1[m][jj];
+            U2[(size_t)j * 64 + m] = t2[m][jj];
+        }
+    }
 }
 
 // One wave's 64 rows x 64 members: the MFMAs of member subtiles j < L (member
@@ -221,28 +241,41 @@ __device__ __forceinline__ double eclip(double x) { return x < -700.0 ? -700.0 :
 
 struct EElem {
     double a, b, dc, f0, P, dPdt, sl, A1, A2, dsig, V;
+    double ia, ib, bdc, c2v0a;  // per-element constants: 1/a, 1/b, b/Dc, 2 V0/a
 };
 
+__device__ __forceinline__ void elem_constants(EElem& e, double V0) {
+    e.ia = 1.0 / e.a;
+    e.ib = 1.0 / e.b;
+    e.bdc = e.b / e.dc;
+    e.c2v0a = 2.0 * V0 * e.ia;
+}
+
+// The aging-law RHS with radiation damping (QDsim_gpu.py:830-891): the same
+// formulas as the dense path's, with the per-element quotients taken once per
+// attempt, one reciprocal of sigma per stage, and exp(-c) as 1/exp(c).
 __device__ __forceinline__ void ens_rhs(const EElem& e, double V0, double eta, const double* y, double* k) {
     const double tn = y[0] * 1e6, t1 = y[1] * 1e6, t2 = y[2] * 1e6, st = y[3];
     const double sg = tn - e.P;
-    const double as = e.a * sg;
-    const double ex = exp(eclip(-st / e.a));
-    const double c1 = eclip(t1 / as), c2 = eclip(t2 / as);
-    const double ep1 = exp(c1), em1 = exp(-c1), ep2 = exp(c2), em2 = exp(-c2);
-    const double ch1 = (ep1 + em1) / 2, ch2 = (ep2 + em2) / 2;
-    const double sh1 = (ep1 - em1) / 2, sh2 = (ep2 - em2) / 2;
-    const double q = 2 * V0 / as;
-    const double dV1dtau = q * ex * ch1, dV2dtau = q * ex * ch2;
-    const double dV1dsig = -2 * V0 * t1 / (e.a * (sg * sg)) * ex * ch1;
-    const double dV2dsig = -2 * V0 * t2 / (e.a * (sg * sg)) * ex * ch2;
-    const double dV1dst = -2 * V0 / e.a * ex * sh1;
-    const double dV2dst = -2 * V0 / e.a * ex * sh2;
-    const double dstdt = e.b / e.dc * (V0 * exp(eclip((e.f0 - st) / e.b)) - e.V);
+    const double isg = 1.0 / sg;
+    const double ias = e.ia * isg;  // 1 / (a sigma)
+    const double ex = exp(eclip(-st * e.ia));
+    const double c1 = eclip(t1 * ias), c2 = eclip(t2 * ias);
+    const double ep1 = exp(c1), ep2 = exp(c2);
+    const double em1 = 1.0 / ep1, em2 = 1.0 / ep2;
+    const double ch1 = (ep1 + em1) * 0.5, ch2 = (ep2 + em2) * 0.5;
+    const double sh1 = (ep1 - em1) * 0.5, sh2 = (ep2 - em2) * 0.5;
+    const double qx = 2 * V0 * ias * ex;  // 2 V0 exp(-psi/a) / (a sigma)
+    const double dV1dtau = qx * ch1, dV2dtau = qx * ch2;
+    const double dV1dsig = -qx * (t1 * isg) * ch1;
+    const double dV2dsig = -qx * (t2 * isg) * ch2;
+    const double dV1dst = -e.c2v0a * ex * sh1;
+    const double dV2dst = -e.c2v0a * ex * sh2;
+    const double dstdt = e.bdc * (V0 * exp(eclip((e.f0 - st) * e.ib)) - e.V);
     const double ds = e.dsig - e.dPdt;
-    k[0] = e.dsig / 1e6;
-    k[1] = (-e.A1 + e.sl - eta * (dV1dsig * ds + dV1dst * dstdt)) / (1.0 + eta * dV1dtau) / 1e6;
-    k[2] = (-e.A2 + e.sl - eta * (dV2dsig * ds + dV2dst * dstdt)) / (1.0 + eta * dV2dtau) / 1e6;
+    k[0] = e.dsig * 1e-6;
+    k[1] = (-e.A1 + e.sl - eta * (dV1dsig * ds + dV1dst * dstdt)) / (1.0 + eta * dV1dtau) * 1e-6;
+    k[2] = (-e.A2 + e.sl - eta * (dV2dsig * ds + dV2dst * dstdt)) / (1.0 + eta * dV2dtau) * 1e-6;
     k[3] = dstdt;
 }
 
@@ -348,6 +381,7 @@ __global__ __launch_bounds__(256) void ens_attempt_kernel(EnsArgs A, int round) 
         e.P = F(E_P); e.dPdt = F(E_DPDT0); e.sl = F(E_SHEAR_LOADING);
         e.A1 = F(E_ADOTV1); e.A2 = F(E_ADOTV2); e.dsig = F(E_DSIGMADT);
         e.V = F(E_SLIPV);
+        elem_constants(e, A.V0);
         const double y[4] = {F(E_TNO), F(E_TT1O), F(E_TT2O), F(E_STATE)};
         double y5[4], y4[4];
         ens_dp45(e, A.V0, A.eta, y, mc[m].h, y5, y4);
@@ -494,7 +528,7 @@ static EnsArgs ens_args(Ensemble* e) {
 static void ens_launch_prep(Ensemble* e, hipStream_t st) {
     const long long items = e->n * e->B;
     double* f = e->f;
-    hipLaunchKernelGGL(ens_prep_u_kernel, dim3((unsigned)((items + 255) / 256)), dim3(256), 0, st,
+    hipLaunchKernelGGL(ens_prep_u_kernel, dim3((unsigned)((e->n + 63) / 64)), dim3(256), 0, st,
                        f + (size_t)E_SLIPV1 * items, f + (size_t)E_SLIPV2 * items, f + (size_t)E_COSR0 * items,
                        f + (size_t)E_SINR0 * items, e->Vpl, e->n, e->B, e->mc, e->ctrl, e->U,
                        e->U + (size_t)e->n * e->B);

@@ -56,7 +56,7 @@ def test_dense_candidates():
 def test_sweep_picks_and_applies_the_fastest(cache):
     obj = FakeTiled(20000, best=2224)
     res = autotune.tune_ensemble(obj)
-    assert res["kc"] == min(res["table"], key=lambda t: t[1])[0]
+    assert res["kc"] == min(res["table"], key=lambda t: t[1])[0]   # deterministic times: finalists agree
     assert obj.kc == res["kc"]
     assert abs(res["kc"] - 2224) <= 2224 // 8   # the candidate nearest the optimum
     d = json.loads(cache.read_text())

@@ -77,7 +77,8 @@ def test_gemm_any_width_matches_fp64(mesh, tmp_path, monkeypatch):
         assert np.sqrt(np.mean(err ** 2)) < np.sqrt(e.kc) * u, (kc, np.sqrt(np.mean(err ** 2)))
         results[e.kc] = got
     res = autotune.tune_ensemble(e)
-    assert e.kc == res["kc"] == min(res["table"], key=lambda t: t[1])[0]
+    assert e.kc == res["kc"]
+    assert res["kc"] in [k for k, _ in sorted(res["table"], key=lambda t: t[1])[:4]]  # a re-timed finalist
     assert autotune.cached(autotune.ensemble_key(e)) == res["kc"]
     e.gemm()
     got = e.gemm_result()
