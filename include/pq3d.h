@@ -235,6 +235,9 @@ int pq3d_lowrank_recompress(long long m, long long n, long long k, const double*
                             long long* r, double* Uout, double* Vout);
 /* stats8: sweeps, launches, pairs evaluated, GPU s, host s, max rank, unconverged, total s */
 int pq3d_aca_stats(void* result, double* stats8);
+/* wall seconds: request assembly, evaluation incl. transfers, host ACA logic,
+ * recompression */
+int pq3d_aca_phases(void* result, double* secs4);
 /* U (m x k column-major) and V (k x n row-major) per block and matrix A1s..Bd, back to back */
 int pq3d_aca_fetch(void* result, double* U, double* V);
 int pq3d_aca_free(void* result);

@@ -246,6 +246,7 @@ def _declare(L):
     _lib._sig(L, "pq3d_aca_run", [vp, ll, lp, ip, lp, ip, d, d, i, ctypes.c_ulonglong, lp, ctypes.POINTER(vp)])
     _lib._sig(L, "pq3d_lowrank_recompress", [ll, ll, ll, dp, dp, d, lp, dp, dp])
     _lib._sig(L, "pq3d_aca_stats", [vp, dp])
+    _lib._sig(L, "pq3d_aca_phases", [vp, dp])
     _lib._sig(L, "pq3d_aca_fetch", [vp, dp, dp])
     _lib._sig(L, "pq3d_aca_free", [vp])
     _lib._sig(L, "pq3d_hmat_create", [ctypes.POINTER(vp), ll, ip, ll, lp, ip, lp, ip, ip, lp, dp, dp, dp, i])
@@ -360,6 +361,8 @@ def aca_blocks(src, blocks, eps=1e-4, recompress_eps=None, max_rank=0, seed=1234
         check(L.pq3d_aca_fetch(res, ptr(U), ptr(V)), "pq3d_aca_fetch")
         stats = np.zeros(8)
         check(L.pq3d_aca_stats(res, ptr(stats)), "pq3d_aca_stats")
+        phases = np.zeros(4)
+        check(L.pq3d_aca_phases(res, ptr(phases)), "pq3d_aca_phases")
     finally:
         L.pq3d_aca_free(res)
     out = []
@@ -376,7 +379,9 @@ def aca_blocks(src, blocks, eps=1e-4, recompress_eps=None, max_rank=0, seed=1234
         out.append(per)
     st = {"sweeps": int(stats[0]), "launches": int(stats[1]), "pairs_evaluated": float(stats[2]),
           "gpu_eval_s": float(stats[3]), "host_s": float(stats[4]), "max_rank": int(stats[5]),
-          "unconverged": int(stats[6])}
+          "unconverged": int(stats[6]),
+          "phase_s": {"requests": float(phases[0]), "evaluation": float(phases[1]), "aca_logic": float(phases[2]),
+                      "recompression": float(phases[3])}}
     return out, st
 
 

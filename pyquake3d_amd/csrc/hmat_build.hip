@@ -626,6 +626,7 @@ struct AcaResult {
     long long nb = 0;
     std::vector<Inst> inst;
     double stats[8] = {0};
+    double phase_s[4] = {0};  // request assembly, evaluation incl. transfers, ACA logic, recompression
 };
 
 }  // namespace pq3d
@@ -792,7 +793,10 @@ int pq3d_aca_run(void* h, long long nb, const long long* row_off, const int* row
     long long sweeps = 0;
     int rc = 0;
     const long long max_sweeps = 4LL * (long long)(max_rank > 0 ? max_rank : 4096) + 64;
+    using clk = std::chrono::steady_clock;
+    auto secs = [](clk::time_point a, clk::time_point b) { return std::chrono::duration<double>(b - a).count(); };
     while (true) {
+        const auto t0 = clk::now();
         reqs.clear();
         key.clear();
         for (auto& s : res->inst) {
@@ -832,25 +836,33 @@ int pq3d_aca_run(void* h, long long nb, const long long* row_off, const int* row
             outs += (long long)popc(r.mask) * d.m * d.n;
         }
         if ((long long)out.size() < outs) out.resize((size_t)outs);
+        const auto t1 = clk::now();
         rc = run_segments(g, d_idx, d_idx + nr, segs, pairs, outs, out.data(), &gpu_ms);
         if (rc) break;
+        const auto t2 = clk::now();
         pairs_total += (double)pairs;
         Results R{&reqs, &segs, out.data()};
         parallel_for((long long)res->inst.size(), [&](long long x) {
             Inst& s = res->inst[(size_t)x];
             if (s.state != ST_DONE && s.nq) consume(s, R, eps);
         });
+        const auto t3 = clk::now();
+        res->phase_s[0] += secs(t0, t1);
+        res->phase_s[1] += secs(t1, t2);
+        res->phase_s[2] += secs(t2, t3);
     }
     (void)hipFree(d_idx);
     if (rc) {
         delete res;
         return rc;
     }
+    const auto t_rc = clk::now();
     if (recompress_eps > 0)
         parallel_for((long long)res->inst.size(), [&](long long x) {
             Inst& s = res->inst[(size_t)x];
             s.k = recompress_factors(s.m, s.n, s.k, s.U, s.V, recompress_eps);
         });
+    res->phase_s[3] = secs(t_rc, clk::now());
     int maxk = 0;
     long long unconv = 0;
     for (auto& s : res->inst) {
@@ -879,6 +891,16 @@ int pq3d_aca_stats(void* r, double* stats8) {
     AcaResult* res = (AcaResult*)r;
     PQ_CHECK(res, "pq3d_aca_stats: null result");
     for (int k = 0; k < 8; ++k) stats8[k] = res->stats[k];
+    return 0;
+}
+
+// Wall seconds of the build's phases: request assembly, evaluation (kernel and
+// transfers), ACA logic on the host threads, recompression.
+int pq3d_aca_phases(void* r, double* secs4) {
+    clear_error();
+    AcaResult* res = (AcaResult*)r;
+    PQ_CHECK(res && secs4, "pq3d_aca_phases: null argument");
+    for (int k = 0; k < 4; ++k) secs4[k] = res->phase_s[k];
     return 0;
 }
 
