@@ -174,6 +174,56 @@ def cpu_baseline_step(cols, mats64, budget_s=15.0, max_steps=40):
                       f"same 20,000-element FP64 matrices, BLAS threads = {threads}, CPU {_cpu_model()}"}
 
 
+def config1_leg(steps=2000, cpu_budget_s=8.0, graph_steps=8):
+    """The reference's CPU-path case at its own size: 80 x 40 km planar fault,
+    40 x 20 squares split into 1,600 TDs, half space, 2,000 adaptive steps.
+    GPU: the plain FP64 dense path over all 2,000 steps (wall clock of the
+    run, host sync at the end).  CPU: the NumPy FP64 restatement of
+    simu_forward on the same matrices for a bounded sample of steps."""
+    from oracle import qdsim_oracle as Q
+    from pyquake3d_amd import _lib, greenfunc, readmsh, synth
+    from pyquake3d_amd.QDsim_gpu import DenseSim
+    nodes_km, tri = synth.planar_fault_mesh(40, 20, 80.0, 40.0)
+    nodes = nodes_km * 1e3
+    ev, xg = readmsh.get_eleVec(nodes, tri, False)
+    cols = synth.friction_params(xg)
+    n = len(tri)
+    s = DenseSim(n, _lib.MAT_F64)
+    s.assemble(*greenfunc.source_vertices(nodes, tri), xg, ev, MU, LAM, True)
+    init_state(s, cols)
+    mats = [s.download_matrix(k) for k in range(6)]
+    s.step(1e-3)
+    _lib.lib().pq3d_synchronize()
+    t0 = time.perf_counter()
+    st, rec = s.run(steps - 1, graph_steps, records=True)
+    wall = time.perf_counter() - t0
+    assert st == 0, f"config-1 run failed with status {st}"
+    sim_years = float(rec[-1, 2]) / (365.25 * 86400)
+    max_v = float(rec[:, 3].max())
+    s.close()
+    out = {"n_elements": n, "steps": steps, "gpu_wall_s": wall, "gpu_steps_per_s": (steps - 1) / wall,
+           "simulated_years": sim_years, "max_slip_rate": max_v, "cpu_baseline": None}
+    if cpu_budget_s > 0:
+        orc = Q.from_init_columns(mats, cols, V0, VPL, MU, CS, False)
+        orc.step(1e-3)
+        dt = 1.5e-3
+        t0 = time.perf_counter()
+        k = 0
+        while k < steps and time.perf_counter() - t0 < cpu_budget_s:
+            _, dt = orc.step(dt)
+            k += 1
+        threads = len(os.sched_getaffinity(0))
+        out["cpu_baseline"] = {"value": k / (time.perf_counter() - t0), "unit": "steps/s", "cores": threads,
+                               "kind": "port",
+                               "sample": f"{k} steps of the NumPy FP64 restatement of simu_forward on the same "
+                                         f"matrices, BLAS threads = {threads}, CPU {_cpu_model()}"}
+    cpu = out["cpu_baseline"]
+    log(f"[bench] config 1 (1,600 TDs): {steps} steps in {wall:.2f} s on the GPU ({out['gpu_steps_per_s']:.0f} "
+        f"steps/s, {sim_years:.1f} years, max V {max_v:.3g} m/s)"
+        + (f"; CPU port {cpu['value']:.1f} steps/s on {cpu['cores']} threads" if cpu else ""))
+    return out
+
+
 def _ref_column(args):
     from oracle import refbind
     P1, P2, P3, xg = args
@@ -459,6 +509,7 @@ def main():
     ap.add_argument("--serve-requests", type=int, default=24, help="0 skips the serving leg")
     ap.add_argument("--serve-rate", type=float, default=1.0, help="Poisson arrival rate, requests/s")
     ap.add_argument("--no-hmat", action="store_true", help="skip the 60k H-matrix leg")
+    ap.add_argument("--no-config1", action="store_true", help="skip the 1,600-element 2,000-step leg")
     ap.add_argument("--hmat-only", action="store_true", help="run only the 60k H-matrix leg (JSON of that leg)")
     ap.add_argument("--hmat-steps", type=int, default=200)
     ap.add_argument("--copy-probe-gib", type=float, default=0.0,
@@ -603,6 +654,9 @@ def main():
             f"p99 {serving['p99_time_to_result_s']:.2f} s, {serving['member_steps_per_s']:.0f} member-steps/s, "
             f"occupancy {serving['mean_occupancy']:.2f}")
 
+    config1 = None if args.no_config1 else config1_leg(graph_steps=args.graph_steps,
+                                                       cpu_budget_s=0.0 if args.no_cpu_baseline else 8.0)
+
     hmat = None
     if not args.no_hmat:
         hmat = hmatrix_leg(steps=args.hmat_steps, warmup=args.warmup, graph_steps=args.graph_steps,
@@ -649,6 +703,7 @@ def main():
         "gf_assembly": gf,
         "p99_solve_latency_s": serving["p99_time_to_result_s"] if serving else None,
         "serving": serving,
+        "config1": config1,
         "hmatrix": hmat,
     }
     print(json.dumps(out))
