@@ -27,7 +27,7 @@ int scratch_get(Scratch& s, size_t bytes, void** out) {
     int dev = 0;
     PQ_HIP(hipGetDevice(&dev));
     if (s.p && (s.bytes < bytes || s.device != dev)) {
-        if (s.device == dev) hipFree(s.p);
+        if (s.device == dev) (void)hipFree(s.p);
         s.p = nullptr;
         s.bytes = 0;
     }

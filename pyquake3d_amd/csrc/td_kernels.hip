@@ -346,7 +346,7 @@ int pq3d_gf_assemble(const double* P1, const double* P2, const double* P3, long 
         set_error("pq3d_gf_assemble: %s", hipGetErrorString(e));
         rc = -1;
     }
-    hipFree(base);
+    (void)hipFree(base);
     return rc;
 }
 
