@@ -235,19 +235,17 @@ __global__ __launch_bounds__(256) void hmat_phase_b_kernel(const BWave* __restri
             const float* D = ddata + it.off[0] + 4 * q;
             const float* x1 = xp + it.c0;
             const float* x2 = xp + n + it.c0;
-            const long long cs = (long long)it.n * msp;  // matrix stride
-            const int nm = fix_tn ? 4 : 6;
-            const int mat[6] = {0, 1, 3, 4, 2, 5};         // Bs, Bd last (skipped with fix_tn)
+            const int nm = fix_tn ? 4 : 6;                 // Bs, Bd last (skipped with fix_tn)
             for (int j0 = g; j0 < it.n; j0 += 2 * G) {     // two columns' loads in flight
                 float4 m[2][6];
                 float a[2], b[2];
 #pragma unroll
                 for (int u = 0; u < 2; ++u) {
                     const int j = j0 + u * G < it.n ? j0 + u * G : j0;
-                    const float* col = D + (long long)j * msp;
+                    const float* col = D + (long long)j * 6 * msp;  // [column][matrix][row]
 #pragma unroll
                     for (int c = 0; c < 6; ++c)
-                        if (c < nm) m[u][c] = *reinterpret_cast<const float4*>(col + mat[c] * cs);
+                        if (c < nm) m[u][c] = *reinterpret_cast<const float4*>(col + c * msp);
                     const bool live = j0 + u * G < it.n;
                     a[u] = live ? x1[j] : 0.f;
                     b[u] = live ? x2[j] : 0.f;
@@ -566,12 +564,16 @@ static int pack(HMat* h, long long n, const int* perm, long long nb, const long 
                 it.off[0] = base;
                 dpack.resize((size_t)(base + 6 * nc * msp), 0.0f);
                 const double* Db = dense + doff[b];
+                // [column][matrix][row]: a column's six matrices are one
+                // contiguous run, A1s, A2s, A1d, A2d first and Bs, Bd last
+                // (the four a fix_tn product reads)
+                static const int pos[6] = {0, 1, 4, 2, 3, 5};
                 for (int c = 0; c < 6; ++c)
                     for (long long jp = 0; jp < nc; ++jp) {
                         const int j = cinv[cinv_off[b] + jp];
                         for (int ip = 0; ip < ms; ++ip) {
                             const int i = rinv[rinv_off[b] + lr0 + ip];
-                            dpack[(size_t)(base + ((long long)c * nc + jp) * msp + ip)] =
+                            dpack[(size_t)(base + (jp * 6 + pos[c]) * msp + ip)] =
                                 (float)Db[(long long)c * m * nc + (long long)j * m + i];
                         }
                     }
