@@ -335,7 +335,10 @@ _GPU_ATTRS = {"Tno_gpu": "Tno", "Tt1o_gpu": "Tt1o", "Tt2o_gpu": "Tt2o", "state_g
 class QDsim:
     """Dense quasi-dynamic earthquake-cycle simulator (QDsim_gpu.py:119-186)."""
 
-    def __init__(self, elelst, nodelst, fnamePara):
+    def __init__(self, elelst, nodelst, fnamePara, device=True):
+        """The reference's constructor.  ``device=False`` (an extension) stops
+        after the host-side initial state (parameters, friction, stresses) and
+        creates no device simulation and no Green's functions."""
         i = fnamePara.rfind("/")
         self.dirname = fnamePara[:i] if i != -1 else fnamePara
         self.Para0 = readPara0(fnamePara)
@@ -373,6 +376,9 @@ class QDsim:
         print("Poissons ratio", possonratio)
         self.Relerrormax1_last = 0
         self.Relerrormax2_last = 0
+        if not device:
+            self.Init_condition()
+            return
         self.sim = DenseSim(len(elelst), self.mat_type)
         self.Init_condition()
         self.calc_corefunc()
