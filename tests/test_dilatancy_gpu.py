@@ -10,6 +10,8 @@
   accepted step) against the NumPy oracle doing the same: step sizes within
   1e-9, P, the profiles and the state within 1e-9 of their scale, dP/dt
   within 1e-9 of the flux-term scale.
+* the reference's own 1,500-step run with dilatancy (golden fixture) against
+  the drop-in QDsim mirror.
 """
 import os
 
@@ -132,3 +134,61 @@ def test_dropin_qdsim_with_dilatancy(tmp_path):
         dttry = dtnext
     assert np.abs(sim0.P - orc.P).max() <= 1e-9 * np.abs(orc.P).max()
     assert np.abs(sim0.Parr - Parr).max() <= 1e-9 * np.abs(Parr).max()
+
+
+def test_mirror_follows_reference_run_with_dilatancy(tmp_path):
+    """The reference's own simu_forward + Calc_P_implicit run on the
+    1,600-element case (tests/golden/make_dilatancy_trajectory.py) against
+    the mirror on bit-identical Green's functions (the reference C++,
+    SHA-256 checked): time, h, dtnext and max slip rate of steps 1-100 within
+    1e-12 and of all 1,500 steps within 1e-7 (measured 3e-16 and 2.6e-9: the
+    run stays aseismic, max V 1.15e-6 m/s over 212 years, so rounding
+    differences grow slowly); the fault pore pressure after step 100 within
+    1e-9 of its scale; the same seismic events (none in this run)."""
+    import sys
+    from oracle import refbind
+    from pyquake3d_amd import readmsh, synth
+    from pyquake3d_amd.QDsim_gpu import QDsim
+    if not refbind.available():
+        pytest.skip("reference C++ (oracle/_ref) not built")
+    golden = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+    sys.path.insert(0, golden)
+    import make_dilatancy_trajectory as DT
+    import make_qdsim_trajectory as G
+    g = dict(np.load(os.path.join(golden, "dilatancy_trajectory.npz"), allow_pickle=False))
+    nodes_km, nodes_m, tri, cols = G.case_inputs()
+    mats = G.traction_matrices(nodes_m, tri, workers=8, context="spawn")
+    assert G.matrices_sha256(mats) == str(g["gf_sha256"])
+    core = tmp_path / "core"
+    core.mkdir()
+    for k, name in enumerate(("A1s", "A2s", "Bs", "A1d", "A2d", "Bd")):
+        np.save(core / (name + ".npy"), mats[k])
+    synth.write_msh(str(tmp_path / "mesh.msh"), nodes_km, tri)
+    synth.write_param_file(str(tmp_path / "param.dat"), cols)
+    (tmp_path / "parameter.txt").write_text(DT.parameter_text(str(core)))
+    nodelst, elelst = readmsh.read_mshV2(str(tmp_path / "mesh.msh"))
+    sim = QDsim(elelst, nodelst, str(tmp_path / "parameter.txt"))
+    assert sim.Ifdila
+    ref = g["records"]
+    rec = np.zeros_like(ref)
+    dt = sim.htry
+    for k in range(len(ref)):
+        h, dtn = sim.simu_forward(dt)
+        rec[k] = (sim.time, h, dtn, sim.maxslipv0)
+        dt = dtn
+        if k + 1 == 100:
+            p100 = np.copy(sim.P)
+    rel = (np.abs(rec - ref) / np.abs(ref)).max(axis=1)
+    print(f"dilatancy run, mirror vs reference: steps 1-100 {rel[:100].max():.2e}, all {rel.max():.2e}; "
+          f"max V {ref[:, 3].max():.4g} / {rec[:, 3].max():.4g}; t_end {ref[-1, 0]:.6e} / {rec[-1, 0]:.6e}",
+          flush=True)
+    assert rel[:100].max() <= 1e-12
+    assert rel.max() <= 1e-7
+    want = g["P_100"]
+    assert np.abs(p100 - want).max() <= 1e-9 * np.abs(want).max()
+    er, ed = G.events(ref), G.events(rec)
+    assert len(ed) == len(er)
+    for a, b in zip(er, ed):
+        print(f"event onset {a[0]:.6e} / {b[0]:.6e} s, peak V {a[1]:.4f} / {b[1]:.4f} m/s", flush=True)
+        assert abs(a[0] - b[0]) <= 1e-3 * a[0]
+        assert abs(a[1] - b[1]) <= 1e-2 * a[1]
