@@ -228,6 +228,23 @@ int pq3d_gf_segments(void* src, const int* rcv_idx, long long nr_idx, const int*
 int pq3d_aca_run(void* src, long long nb, const long long* row_off, const int* row_idx, const long long* col_off,
                  const int* col_idx, double eps, double recompress_eps, int max_rank, unsigned long long seed,
                  long long* ranks, void** result);
+/* the same ACA (same seeds, same stopping rules) with the pivot logic on the
+ * device: one wave per (block, matrix), one count read back per sweep instead
+ * of every evaluated row and column, and the recompression (same algorithm)
+ * on the device too; the final factors stay in HBM until pq3d_aca_fetch
+ * copies them out.  Factors of at most kcap terms live on the device
+ * (kcap <= 0: 48; at most 64); an instance needing more is redone and
+ * recompressed by the host loop from its own seed.  Sums run in another order
+ * than the host loop's, so pivots may differ where residual entries nearly
+ * tie; the stopping rules and their tolerances are the same.  Results as
+ * pq3d_aca_run. */
+int pq3d_aca_run_device(void* src, long long nb, const long long* row_off, const int* row_idx,
+                        const long long* col_off, const int* col_idx, double eps, double recompress_eps,
+                        int max_rank, unsigned long long seed, int kcap, long long* ranks, void** result);
+/* info4 of a pq3d_aca_run_device result: blocks redone by the host loop, wall
+ * seconds of set-up, sweeps, factor readback in pq3d_aca_fetch (zeros for
+ * pq3d_aca_run) */
+int pq3d_aca_device_info(void* result, double* info4);
 /* host only: U (m x k column-major) V (k x n row-major) recompressed to
  * relative Frobenius tolerance eps; rank in *r, factors in Uout / Vout
  * (the reference's SVD_recompress, src/Hmatrix.py:116-141) */

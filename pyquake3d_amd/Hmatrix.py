@@ -244,6 +244,9 @@ def _declare(L):
     _lib._sig(L, "pq3d_gfsrc_destroy", [vp])
     _lib._sig(L, "pq3d_gf_segments", [vp, ip, ll, ip, ll, ll, lp, dp])
     _lib._sig(L, "pq3d_aca_run", [vp, ll, lp, ip, lp, ip, d, d, i, ctypes.c_ulonglong, lp, ctypes.POINTER(vp)])
+    _lib._sig(L, "pq3d_aca_run_device", [vp, ll, lp, ip, lp, ip, d, d, i, ctypes.c_ulonglong, i, lp,
+                                         ctypes.POINTER(vp)])
+    _lib._sig(L, "pq3d_aca_device_info", [vp, dp])
     _lib._sig(L, "pq3d_lowrank_recompress", [ll, ll, ll, dp, dp, d, lp, dp, dp])
     _lib._sig(L, "pq3d_aca_stats", [vp, dp])
     _lib._sig(L, "pq3d_aca_phases", [vp, dp])
@@ -333,11 +336,15 @@ class GFSource:
             pass
 
 
-def aca_blocks(src, blocks, eps=1e-4, recompress_eps=None, max_rank=0, seed=12345):
+def aca_blocks(src, blocks, eps=1e-4, recompress_eps=None, max_rank=0, seed=12345, device=False, kcap=0):
     """Adaptive cross approximation of the six matrices of every block in
     ``blocks`` (lists of row and column ids), recompressed to
     ``recompress_eps`` (default eps; 0 keeps the ACA factors).  Returns, per
-    block, a list of six (U, V) pairs and the run statistics."""
+    block, a list of six (U, V) pairs and the run statistics.
+
+    device=True runs the pivot logic on the GPU (``pq3d_aca_run_device``, one
+    wave per block and matrix, factors of up to ``kcap`` terms on the device,
+    default 48); a matrix needing more terms is redone by the host loop."""
     rc_eps = eps if recompress_eps is None else recompress_eps
     nb = len(blocks)
     rows = [_i32(b[0]) for b in blocks]
@@ -349,8 +356,13 @@ def aca_blocks(src, blocks, eps=1e-4, recompress_eps=None, max_rank=0, seed=1234
     ranks = np.zeros((nb, 6), dtype=np.int64)
     res = ctypes.c_void_p()
     L = _lib.lib()
-    check(L.pq3d_aca_run(src.h, nb, _lp(roff), _ip(ridx), _lp(coff), _ip(cidx), float(eps), float(rc_eps),
-                         int(max_rank), int(seed), _lp(ranks), ctypes.byref(res)), "pq3d_aca_run")
+    if device:
+        check(L.pq3d_aca_run_device(src.h, nb, _lp(roff), _ip(ridx), _lp(coff), _ip(cidx), float(eps),
+                                    float(rc_eps), int(max_rank), int(seed), int(kcap), _lp(ranks),
+                                    ctypes.byref(res)), "pq3d_aca_run_device")
+    else:
+        check(L.pq3d_aca_run(src.h, nb, _lp(roff), _ip(ridx), _lp(coff), _ip(cidx), float(eps), float(rc_eps),
+                             int(max_rank), int(seed), _lp(ranks), ctypes.byref(res)), "pq3d_aca_run")
     try:
         m = np.diff(roff)
         n = np.diff(coff)
@@ -363,6 +375,8 @@ def aca_blocks(src, blocks, eps=1e-4, recompress_eps=None, max_rank=0, seed=1234
         check(L.pq3d_aca_stats(res, ptr(stats)), "pq3d_aca_stats")
         phases = np.zeros(4)
         check(L.pq3d_aca_phases(res, ptr(phases)), "pq3d_aca_phases")
+        dinfo = np.zeros(4)
+        check(L.pq3d_aca_device_info(res, ptr(dinfo)), "pq3d_aca_device_info")
     finally:
         L.pq3d_aca_free(res)
     out = []
@@ -379,9 +393,12 @@ def aca_blocks(src, blocks, eps=1e-4, recompress_eps=None, max_rank=0, seed=1234
         out.append(per)
     st = {"sweeps": int(stats[0]), "launches": int(stats[1]), "pairs_evaluated": float(stats[2]),
           "gpu_eval_s": float(stats[3]), "host_s": float(stats[4]), "max_rank": int(stats[5]),
-          "unconverged": int(stats[6]),
+          "unconverged": int(stats[6]), "pivoting": "device" if device else "host",
+          "redone_blocks": int(dinfo[0]),
           "phase_s": {"requests": float(phases[0]), "evaluation": float(phases[1]), "aca_logic": float(phases[2]),
                       "recompression": float(phases[3])}}
+    if device:
+        st["device_s"] = {"setup": float(dinfo[1]), "sweeps": float(dinfo[2]), "readback": float(dinfo[3])}
     return out, st
 
 
@@ -410,6 +427,8 @@ def recompress(U, V, eps):
 # ----------------------------------------------------------------- block tree
 class BlockTree:
     """Block tree with the reference's attributes and methods (Hmatrix.py:258-2070)."""
+
+    aca_device = True  # ACA pivot logic and recompression on the GPU (False: pivot logic on host threads)
 
     def __init__(self, root_block, nodelst, elelst, eleVec, mu_, lambda_, xg, halfspace_jud, mini_leaf,
                  cluster_root=None):
@@ -539,7 +558,8 @@ class BlockTree:
         t0 = time.perf_counter()
         if lowrank:
             out, st = aca_blocks(self.src, [(b.row_cluster, b.col_cluster) for b in lowrank], eps=self.aca_eps,
-                                 recompress_eps=self.aca_eps if self.recompress else 0.0)
+                                 recompress_eps=self.aca_eps if self.recompress else 0.0,
+                                 device=self.aca_device)
             t1 = time.perf_counter()
             for b, per in zip(lowrank, out):
                 dS, dD = {}, {}
@@ -677,9 +697,11 @@ class DeviceHmatrix:
 
 
 def build(nodes, tri, mu, lam, halfspace=True, mini_leaf=32, min_size=16, eta=2.0, eps=1e-4,
-          recompress_factors=True):
+          recompress_factors=True, aca_device=True):
     """Mesh -> compressed ``BlockTree`` with ``blocks_to_process`` filled.
-    ``nodes`` in metres, ``tri`` 1-based (the reader's conventions)."""
+    ``nodes`` in metres, ``tri`` 1-based (the reader's conventions);
+    aca_device: pivot logic and recompression of the ACA on the GPU (False:
+    pivot logic on host threads)."""
     eleVec, xg = readmsh.get_eleVec(nodes, tri, False)
     para = {"Shear modulus": mu, "Lame constants": lam, "Half space": bool(halfspace)}
     t0 = time.perf_counter()
@@ -688,5 +710,6 @@ def build(nodes, tri, mu, lam, halfspace=True, mini_leaf=32, min_size=16, eta=2.
     tree.build_stats["tree_s"] = time.perf_counter() - t0
     tree.aca_eps = eps
     tree.recompress = recompress_factors
+    tree.aca_device = aca_device
     tree.calc_Hmatrix_ACA()
     return tree

@@ -25,6 +25,7 @@
 
 #include <algorithm>
 #include <chrono>
+#include <memory>
 #include <cmath>
 #include <cstdint>
 #include <thread>
@@ -627,7 +628,999 @@ struct AcaResult {
     std::vector<Inst> inst;
     double stats[8] = {0};
     double phase_s[4] = {0};  // request assembly, evaluation incl. transfers, ACA logic, recompression
+    // device path: blocks redone by the host loop (an instance reached kcap),
+    // seconds of set-up (instances, buffers, uploads), sweeps, factor readback
+    double dev[4] = {0};
+    // device path: final factors in the fetch layout; instances whose factors
+    // live on the host (redone there) are marked in host_inst
+    double* dU = nullptr;
+    double* dV = nullptr;
+    long long usz = 0, vsz = 0;
+    std::vector<unsigned char> host_inst;
+    ~AcaResult() {
+        if (dU) (void)hipFree(dU);
+        if (dV) (void)hipFree(dV);
+    }
 };
+
+// Instances of nb blocks (six matrices each) with their first requests: the
+// reference row and column of block b, drawn from the seed and bid[b] (the
+// block's id in the caller's numbering; b when bid is null).
+static void aca_init_instances(long long nb, const long long* row_off, const long long* col_off,
+                               const long long* bid, int max_rank, uint64_t seed, std::vector<Inst>& inst) {
+    inst.assign((size_t)nb * 6, Inst());
+    for (long long b = 0; b < nb; ++b) {
+        const int m = (int)(row_off[b + 1] - row_off[b]), n = (int)(col_off[b + 1] - col_off[b]);
+        const long long id = bid ? bid[b] : b;
+        uint64_t bs = seed ^ (0x51ED270B27A5ull * (uint64_t)(id + 1));
+        const int i0 = (int)(splitmix(bs) % (uint64_t)m), j0 = (int)(splitmix(bs) % (uint64_t)n);
+        for (int c = 0; c < 6; ++c) {
+            Inst& s = inst[(size_t)b * 6 + c];
+            s.b = (int)b;
+            s.c = c;
+            s.m = m;
+            s.n = n;
+            s.kmax = std::min(m, n);
+            if (max_rank > 0) s.kmax = std::min(s.kmax, max_rank);
+            s.ur.assign(m, 0);
+            s.uc.assign(n, 0);
+            s.Rr.assign(n, 0.0);
+            s.Rc.assign(m, 0.0);
+            s.rng = bs ^ (0x9E37ull * (uint64_t)(c + 1));
+            // all six matrices of a block start from the same reference row/column
+            s.iref = i0;
+            s.jref = j0;
+            push_req(s, 0, i0);
+            push_req(s, 1, j0);
+            s.state = ST_REFS;
+        }
+    }
+}
+
+struct AcaCounters {
+    long long sweeps = 0;
+    double pairs = 0, gpu_ms = 0;
+    double phase_s[3] = {0, 0, 0};
+};
+
+// The sweeps of the host-side pivoting: every sweep evaluates the pending
+// requests of all instances in one launch and advances each instance.
+// d_idx: row ids (row_off[nb] of them) then column ids on the device.
+static int aca_host_loop(GFSource* g, const int* d_idx, const long long* row_off, const long long* col_off,
+                         long long nb, double eps, int max_rank, std::vector<Inst>& inst, AcaCounters& cnt) {
+    const long long nr = row_off[nb];
+    std::vector<Req> reqs;
+    std::vector<SegDesc> segs;
+    std::vector<double> out;
+    std::unordered_map<uint64_t, int> key;
+    const long long max_sweeps = 4LL * (long long)(max_rank > 0 ? max_rank : 4096) + 64;
+    using clk = std::chrono::steady_clock;
+    auto secs = [](clk::time_point a, clk::time_point b) { return std::chrono::duration<double>(b - a).count(); };
+    while (true) {
+        const auto t0 = clk::now();
+        reqs.clear();
+        key.clear();
+        for (auto& s : inst) {
+            if (s.state == ST_DONE) continue;
+            for (int t = 0; t < s.nq; ++t) {
+                const uint64_t kk = ((uint64_t)s.b << 33) | ((uint64_t)s.qt[t] << 32) | (uint32_t)s.qi[t];
+                auto it = key.find(kk);
+                int q;
+                if (it == key.end()) {
+                    q = (int)reqs.size();
+                    key.emplace(kk, q);
+                    reqs.push_back(Req{s.b, s.qt[t], s.qi[t], 0});
+                } else {
+                    q = it->second;
+                }
+                reqs[q].mask |= (s.scale < 0 ? 0x3f : (1 << s.c));
+                s.qn[t] = q;
+            }
+        }
+        if (reqs.empty()) return 0;
+        if (++cnt.sweeps > max_sweeps) {
+            set_error("pq3d_aca_run: no convergence after %lld sweeps", cnt.sweeps);
+            return -3;
+        }
+        segs.resize(reqs.size());
+        long long pairs = 0, outs = 0;
+        for (size_t q = 0; q < reqs.size(); ++q) {
+            const Req& r = reqs[q];
+            const long long ro = row_off[r.b], co = col_off[r.b];
+            const int m = (int)(row_off[r.b + 1] - ro), n = (int)(col_off[r.b + 1] - co);
+            SegDesc d;
+            if (r.type == 0) d = SegDesc{pairs, outs, (int)(ro + r.idx), 1, (int)co, n, r.mask, 0};
+            else d = SegDesc{pairs, outs, (int)ro, m, (int)(co + r.idx), 1, r.mask, 0};
+            segs[q] = d;
+            pairs += (long long)d.m * d.n;
+            outs += (long long)popc(r.mask) * d.m * d.n;
+        }
+        if ((long long)out.size() < outs) out.resize((size_t)outs);
+        const auto t1 = clk::now();
+        int rc = run_segments(g, d_idx, d_idx + nr, segs, pairs, outs, out.data(), &cnt.gpu_ms);
+        if (rc) return rc;
+        const auto t2 = clk::now();
+        cnt.pairs += (double)pairs;
+        Results R{&reqs, &segs, out.data()};
+        parallel_for((long long)inst.size(), [&](long long x) {
+            Inst& s = inst[(size_t)x];
+            if (s.state != ST_DONE && s.nq) consume(s, R, eps);
+        });
+        const auto t3 = clk::now();
+        cnt.phase_s[0] += secs(t0, t1);
+        cnt.phase_s[1] += secs(t1, t2);
+        cnt.phase_s[2] += secs(t2, t3);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// ACA with the pivoting on the device.  The same algorithm as consume /
+// decide / add_term above, one wave per (block, matrix).  Each sweep:
+// aca_dev_logic_kernel consumes the instance's last evaluation and issues its
+// next requests; aca_dev_scan_kernel numbers the requested pairs; and
+// aca_dev_eval_kernel evaluates them, one pair per lane, straight into the
+// requesting instance's buffers.  The host reads one count per sweep.  The
+// factors of an instance hold at most kcap terms; an instance that would
+// need more is marked and redone by the host loop.
+// ---------------------------------------------------------------------------
+struct DAca {
+    int b, c, m, n;
+    int kmax, kcap, k, state;
+    int iref, jref, istar, jstar;
+    int tries, nq, need_rr, need_rc;
+    int lead_row, pending_conv, converged, capped;
+    int first, pad0;
+    int qt[2], qi[2];
+    double norm2, scale;
+    unsigned long long rng, scale_bits;
+    long long offU, offV, offM, offN, offFm, offFn;
+};
+
+__device__ __forceinline__ unsigned long long dsplitmix(unsigned long long& s) {
+    unsigned long long z = (s += 0x9E3779B97F4A7C15ull);
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return z ^ (z >> 31);
+}
+
+// lane 0's value in every lane (a scalar register)
+__device__ __forceinline__ double wave_uniform(double v) {
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_readfirstlane((int)(b & 0xffffffffll));
+    const int hi = __builtin_amdgcn_readfirstlane((int)(b >> 32));
+    return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+
+// sum over the wave, the same bits in every lane.  The butterfly alone is not
+// enough: the compiler may fuse a caller's product into its first add
+// (fma(u_i, u_i, u_j^2) in lane i, fma(u_j, u_j, u_i^2) in lane j), and a
+// branch on such a sum would split the wave; so lane 0's sum is broadcast.
+__device__ __forceinline__ double wsum(double v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return wave_uniform(v);
+}
+
+__device__ __forceinline__ int wmin_int(int v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        const int w = __shfl_xor(v, o, 64);
+        v = w < v ? w : v;
+    }
+    return __builtin_amdgcn_readfirstlane(v);  // same in every lane; now known to be
+}
+
+// argmax over unused entries with the host loop's semantics: the first
+// maximum of |x| in index order; a NaN after the first valid entry wins.
+__device__ int dargmax_unused(const double* x, const unsigned char* used, int len, double* best) {
+    const int lane = threadIdx.x & 63;
+    double bv = -1.0;
+    int bi = INT_MAX, fv = INT_MAX;
+    bool has_nan = false;
+    for (int t = lane; t < len; t += 64) {
+        if (used[t]) continue;
+        const double a = fabs(x[t]);
+        if (a != a) {
+            has_nan = true;
+            continue;
+        }
+        if (t < fv) fv = t;
+        if (a > bv) {
+            bv = a;
+            bi = t;
+        }
+    }
+    // lane-wise first maxima -> wave maximum, smallest index on ties
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        const double ov = __shfl_xor(bv, o, 64);
+        const int oi = __shfl_xor(bi, o, 64);
+        if (ov > bv || (ov == bv && oi < bi)) {
+            bv = ov;
+            bi = oi;
+        }
+    }
+    fv = wmin_int(fv);
+    if (__any(has_nan) && fv != INT_MAX) {
+        int ni = INT_MAX;
+        for (int t = lane; t < len; t += 64)
+            if (!used[t] && t > fv && x[t] != x[t] && t < ni) ni = t;
+        ni = wmin_int(ni);
+        if (ni != INT_MAX) {
+            *best = __longlong_as_double(0x7ff8000000000000ll);
+            return ni;
+        }
+    }
+    bi = __builtin_amdgcn_readfirstlane(bi);  // the reductions leave every lane equal
+    bv = wave_uniform(bv);
+    if (bi == INT_MAX) {
+        *best = 0.0;
+        return -1;
+    }
+    *best = bv;
+    return bi;
+}
+
+__device__ int dpick_unused(DAca& s, const unsigned char* used, int len) {
+    const int lane = threadIdx.x & 63;
+    const int start = (int)(dsplitmix(s.rng) % (unsigned long long)len);
+    for (int base = 0; base < len; base += 64) {
+        const int t = base + lane;
+        const bool ok = t < len && !used[(start + t) % len];
+        const unsigned long long bal = __ballot(ok);
+        if (bal) return __builtin_amdgcn_readfirstlane((start + base + __ffsll((long long)bal) - 1) % len);
+    }
+    return -1;
+}
+
+struct DView {  // the instance's buffers
+    double *U, *V, *Rc, *col, *rawC, *Rr, *r, *rawR;
+    unsigned char *ur, *uc;
+};
+
+__device__ __forceinline__ DView dview(const DAca& s, double* dbuf, unsigned char* fbuf) {
+    DView v;
+    v.U = dbuf + s.offU;
+    v.V = dbuf + s.offV;
+    v.Rc = dbuf + s.offM;
+    v.col = v.Rc + s.m;
+    v.rawC = v.Rc + 2 * (long long)s.m;
+    v.Rr = dbuf + s.offN;
+    v.r = v.Rr + s.n;
+    v.rawR = v.Rr + 2 * (long long)s.n;
+    v.ur = fbuf + s.offFm;
+    v.uc = fbuf + s.offFn;
+    return v;
+}
+
+__device__ __forceinline__ void dset_flag(unsigned char* f, int i) {
+    if ((threadIdx.x & 63) == 0) f[i] = 1;
+    __threadfence_block();
+}
+
+__device__ void dresidual_row(const DAca& s, const DView& w, const double* raw, int i, double* out) {
+    for (int j = threadIdx.x & 63; j < s.n; j += 64) {
+        double v = raw[j];
+        for (int l = 0; l < s.k; ++l) {
+            const double a = w.U[(long long)l * s.m + i];
+            if (a == 0.0) continue;
+            v -= a * w.V[(long long)l * s.n + j];
+        }
+        out[j] = v;
+    }
+    __threadfence_block();
+}
+
+__device__ void dresidual_col(const DAca& s, const DView& w, const double* raw, int j, double* out) {
+    for (int i = threadIdx.x & 63; i < s.m; i += 64) {
+        double v = raw[i];
+        for (int l = 0; l < s.k; ++l) {
+            const double a = w.V[(long long)l * s.n + j];
+            if (a == 0.0) continue;
+            v -= a * w.U[(long long)l * s.m + i];
+        }
+        out[i] = v;
+    }
+    __threadfence_block();
+}
+
+__device__ __forceinline__ void dpush(DAca& s, int type, int idx) {
+    if (s.nq == 0) {  // constant indices keep the instance in registers
+        s.qt[0] = type;
+        s.qi[0] = idx;
+    } else {
+        s.qt[1] = type;
+        s.qi[1] = idx;
+    }
+    ++s.nq;
+}
+
+__device__ void dstart_refs(DAca& s, const DView& w) {
+    if (s.need_rr) {
+        s.iref = dpick_unused(s, w.ur, s.m);
+        if (s.iref < 0) {
+            s.need_rr = 0;
+            for (int j = threadIdx.x & 63; j < s.n; j += 64) w.Rr[j] = 0.0;
+            __threadfence_block();
+        } else {
+            dpush(s, 0, s.iref);
+        }
+    }
+    if (s.need_rc) {
+        s.jref = dpick_unused(s, w.uc, s.n);
+        if (s.jref < 0) {
+            s.need_rc = 0;
+            for (int i = threadIdx.x & 63; i < s.m; i += 64) w.Rc[i] = 0.0;
+            __threadfence_block();
+        } else {
+            dpush(s, 1, s.jref);
+        }
+    }
+    s.state = s.nq ? ST_REFS : ST_LEAD;
+}
+
+__device__ void ddecide(DAca& s, const DView& w, double eps) {
+    for (int guard = 0; guard < 16; ++guard) {
+        const double tiny = 1e-12 * s.scale;
+        const double rms = sqrt(s.norm2 / ((double)s.m * (double)s.n));
+        double ar = 0, ac = 0;
+        const int jc = dargmax_unused(w.Rr, w.uc, s.n, &ar);
+        const int ic = dargmax_unused(w.Rc, w.ur, s.m, &ac);
+        if (s.pending_conv) {
+            s.pending_conv = 0;
+            if (ar <= eps * rms + tiny && ac <= eps * rms + tiny) {
+                s.converged = 1;
+                s.state = ST_DONE;
+                return;
+            }
+        }
+        if (s.k >= s.kmax || (jc < 0 && ic < 0)) {
+            s.converged = (jc < 0 && ic < 0) || s.k >= min(s.m, s.n);
+            s.state = ST_DONE;
+            return;
+        }
+        if (s.k >= s.kcap) {  // the host loop takes over
+            s.capped = 1;
+            s.state = ST_DONE;
+            return;
+        }
+        if (!(ar > tiny) && !(ac > tiny)) {
+            if (++s.tries > 3) {
+                s.converged = 1;
+                s.state = ST_DONE;
+                return;
+            }
+            if (s.iref >= 0) dset_flag(w.ur, s.iref);
+            if (s.jref >= 0) dset_flag(w.uc, s.jref);
+            s.need_rr = s.need_rc = 1;
+            dstart_refs(s, w);
+            if (s.state == ST_LEAD) continue;
+            return;
+        }
+        if (ac > ar) {
+            s.lead_row = 1;
+            s.istar = ic;
+            dpush(s, 0, ic);
+        } else {
+            s.lead_row = 0;
+            s.jstar = jc;
+            dpush(s, 1, jc);
+        }
+        s.state = ST_LEAD;
+        return;
+    }
+    s.state = ST_DONE;
+}
+
+__device__ void dadd_term(DAca& s, const DView& w, double eps) {
+    const int lane = threadIdx.x & 63;
+    const double delta = w.r[s.jstar];
+    const double tiny = 1e-12 * s.scale;
+    dset_flag(w.ur, s.istar);
+    dset_flag(w.uc, s.jstar);
+    if (!(fabs(delta) > tiny)) {
+        ddecide(s, w, eps);
+        return;
+    }
+    const int m = s.m, n = s.n, k = s.k;
+    double* u = w.U + (long long)k * m;
+    double* v = w.V + (long long)k * n;
+    const double inv = 1.0 / delta;
+    double uu = 0, vv = 0;
+    for (int i = lane; i < m; i += 64) {
+        const double x = w.col[i];
+        u[i] = x;
+        uu += x * x;
+    }
+    for (int j = lane; j < n; j += 64) {
+        const double x = w.r[j] * inv;
+        v[j] = x;
+        vv += x * x;
+    }
+    __threadfence_block();
+    uu = wsum(uu);
+    vv = wsum(vv);
+    double cross = 0;
+    for (int l = 0; l < k; ++l) {
+        const double* ul = w.U + (long long)l * m;
+        const double* vl = w.V + (long long)l * n;
+        double a = 0, b = 0;
+        for (int i = lane; i < m; i += 64) a += ul[i] * u[i];
+        for (int j = lane; j < n; j += 64) b += vl[j] * v[j];
+        cross += wsum(a) * wsum(b);
+    }
+    s.norm2 = fmax(0.0, s.norm2 + 2.0 * cross + uu * vv);
+    s.k = k + 1;
+    if (s.iref >= 0) {
+        const double a = u[s.iref];
+        for (int j = lane; j < n; j += 64) w.Rr[j] -= a * v[j];
+        if (w.ur[s.iref]) s.need_rr = 1;
+    }
+    if (s.jref >= 0) {
+        const double a = v[s.jref];
+        for (int i = lane; i < m; i += 64) w.Rc[i] -= a * u[i];
+        if (w.uc[s.jref]) s.need_rc = 1;
+    }
+    __threadfence_block();
+    const double step = sqrt(uu * vv);
+    const double floor_ = 1e-9 * s.scale * sqrt((double)m * (double)n);
+    s.pending_conv = step <= eps * sqrt(s.norm2) || step <= floor_;
+    s.tries = 0;
+    if (s.need_rr || s.need_rc) {
+        dstart_refs(s, w);
+        if (s.state == ST_LEAD) ddecide(s, w, eps);
+    } else {
+        ddecide(s, w, eps);
+    }
+}
+
+// one wave per instance: consume the last evaluation, issue the next requests
+__global__ __launch_bounds__(256) void aca_dev_logic_kernel(DAca* __restrict__ inst, long long ni,
+                                                            double* __restrict__ dbuf,
+                                                            unsigned char* __restrict__ fbuf, double eps,
+                                                            long long* __restrict__ cnt) {
+    // the wave's instance, held in scalar registers
+    const long long x = __builtin_amdgcn_readfirstlane((int)(blockIdx.x * 4 + (threadIdx.x >> 6)));
+    if (x >= ni) return;  // whole wave
+    DAca s = inst[x];
+    if (s.state == ST_DONE || s.nq == 0) {
+        if ((threadIdx.x & 63) == 0) cnt[x] = 0;
+        return;
+    }
+    const DView w = dview(s, dbuf, fbuf);
+    const int nq = s.nq;
+    const int qt0 = s.qt[0], qi0 = s.qi[0], qt1 = s.qt[1], qi1 = s.qi[1];
+    s.nq = 0;
+    if (s.first) {
+        s.scale = __longlong_as_double((long long)s.scale_bits);
+        s.first = 0;
+    }
+    double best = 0;
+    switch (s.state) {
+        case ST_REFS:
+            for (int t = 0; t < nq; ++t) {
+                const int qt = t ? qt1 : qt0, qi = t ? qi1 : qi0;
+                if (qt == 0) {
+                    dresidual_row(s, w, w.rawR, qi, w.Rr);
+                    s.need_rr = 0;
+                } else {
+                    dresidual_col(s, w, w.rawC, qi, w.Rc);
+                    s.need_rc = 0;
+                }
+            }
+            ddecide(s, w, eps);
+            break;
+        case ST_LEAD:
+            if (s.lead_row) {
+                dresidual_row(s, w, w.rawR, s.istar, w.r);
+                s.jstar = dargmax_unused(w.r, w.uc, s.n, &best);
+                if (s.jstar < 0 || !(best > 1e-12 * s.scale)) {
+                    dset_flag(w.ur, s.istar);
+                    ddecide(s, w, eps);
+                    break;
+                }
+                dpush(s, 1, s.jstar);
+            } else {
+                dresidual_col(s, w, w.rawC, s.jstar, w.col);
+                s.istar = dargmax_unused(w.col, w.ur, s.m, &best);
+                if (s.istar < 0 || !(best > 1e-12 * s.scale)) {
+                    dset_flag(w.uc, s.jstar);
+                    ddecide(s, w, eps);
+                    break;
+                }
+                dpush(s, 0, s.istar);
+            }
+            s.state = ST_FOLLOW;
+            break;
+        case ST_FOLLOW:
+            if (s.lead_row) dresidual_col(s, w, w.rawC, s.jstar, w.col);
+            else dresidual_row(s, w, w.rawR, s.istar, w.r);
+            dadd_term(s, w, eps);
+            break;
+        default:
+            break;
+    }
+    if (s.state == ST_DONE) s.nq = 0;
+    long long c = 0;
+    if (s.nq > 0) c += s.qt[0] == 0 ? s.n : s.m;
+    if (s.nq > 1) c += s.qt[1] == 0 ? s.n : s.m;
+    if ((threadIdx.x & 63) == 0) {
+        inst[x] = s;
+        cnt[x] = c;
+    }
+}
+
+// exclusive prefix sum of cnt[0, n) into off[0, n], off[n] = total; one workgroup
+__global__ __launch_bounds__(1024) void aca_dev_scan_kernel(const long long* __restrict__ cnt, long long n,
+                                                            long long* __restrict__ off) {
+    __shared__ long long part[1024];
+    const int t = threadIdx.x;
+    const long long per = (n + 1023) / 1024;
+    const long long lo = t * per, hi = lo + per < n ? lo + per : n;
+    long long sum = 0;
+    for (long long i = lo; i < hi; ++i) sum += cnt[i];
+    part[t] = sum;
+    __syncthreads();
+    for (int o = 1; o < 1024; o <<= 1) {  // inclusive scan of the partial sums
+        const long long v = t >= o ? part[t - o] : 0;
+        __syncthreads();
+        part[t] += v;
+        __syncthreads();
+    }
+    long long run = t ? part[t - 1] : 0;
+    for (long long i = lo; i < hi; ++i) {
+        off[i] = run;
+        run += cnt[i];
+    }
+    if (t == 1023) off[n] = part[1023];
+}
+
+// one pair per lane: the requested values of matrix c go to the instance's
+// raw row/column buffer; an instance's first evaluation also takes the largest
+// finite |entry| over all six matrices (the block's scale)
+__global__ __launch_bounds__(256) void aca_dev_eval_kernel(const SrcGeom<2>* __restrict__ geo,
+                                                           const double* __restrict__ xg,
+                                                           const double* __restrict__ ev,
+                                                           const int* __restrict__ ridx, const int* __restrict__ sidx,
+                                                           const long long* __restrict__ row_off,
+                                                           const long long* __restrict__ col_off,
+                                                           DAca* __restrict__ inst, const long long* __restrict__ off,
+                                                           long long ni, double mu, double lam, double nu,
+                                                           int halfspace, double* __restrict__ dbuf) {
+    const long long total = off[ni];
+    const long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= total) return;
+    long long lo = 0, hi = ni - 1;  // last instance with off <= q (it has pairs)
+    while (lo < hi) {
+        const long long mid = (lo + hi + 1) >> 1;
+        if (off[mid] <= q) lo = mid;
+        else hi = mid - 1;
+    }
+    const DAca& s = inst[lo];
+    long long p = q - off[lo];
+    const long long len0 = s.qt[0] == 0 ? s.n : s.m;
+    const int t = p < len0 ? 0 : 1;
+    if (t) p -= len0;
+    const int type = s.qt[t], idx = s.qi[t];
+    const int i = type == 0 ? idx : (int)p, j = type == 0 ? (int)p : idx;
+    const int r = ridx[row_off[s.b] + i], src = sidx[col_off[s.b] + j];
+    const double X = xg[3 * (long long)r + 0], Y = xg[3 * (long long)r + 1], Z = xg[3 * (long long)r + 2];
+    double e[9];
+#pragma unroll
+    for (int k = 0; k < 9; ++k) e[k] = ev[9 * (long long)r + k];
+    double G[2][9];
+    td_pair_grad<double, 2, false>(geo[src], X, Y, Z, nu, halfspace != 0, G);
+    double vals[6];
+#pragma unroll
+    for (int b = 0; b < 2; ++b) {
+        double eps[6], sig[6], tr[3];
+        grad_to_strain_stress<double>(G[b], mu, lam, eps, sig);
+        stress_to_traction<double>(sig, e, tr);
+#pragma unroll
+        for (int k = 0; k < 3; ++k) vals[b * 3 + k] = tr[k];
+    }
+    double v = vals[0];
+#pragma unroll
+    for (int k = 1; k < 6; ++k)
+        if (k == s.c) v = vals[k];
+    if (type == 0) dbuf[s.offN + 2 * (long long)s.n + p] = v;   // rawR
+    else dbuf[s.offM + 2 * (long long)s.m + p] = v;             // rawC
+    if (s.first) {
+        double mx = 0;
+#pragma unroll
+        for (int k = 0; k < 6; ++k) {
+            const double a = fabs(vals[k]);
+            if (a == a && a > mx) mx = a;
+        }
+        atomicMax(&inst[lo].scale_bits, (unsigned long long)__double_as_longlong(mx));
+    }
+}
+
+// ---- recompression on the device: the algorithm of recompress_factors
+// (MGS2 QR of U and V^T, core Ru Rv^T, one-sided cyclic Jacobi SVD, Frobenius
+// truncation), one wave per (block, matrix); sums run across the wave.
+
+// MGS2 QR (the algorithm of qr_mgs2): Q (rows x k, column j at j*rows) ->
+// orthonormal columns in place, R (k x k, R[i*k + j]) upper triangular.
+// One wave, the column being orthogonalized held in registers (NV values per
+// lane, rows <= 64 NV) and the next basis column loaded while the
+// current one is applied: one pass over Q's columns per step instead of four.
+template <int NV>
+__device__ void dqr_mgs2_regs(int rows, int k, double* Q, double* R) {
+    const int lane = threadIdx.x & 63;
+    for (int e = lane; e < k * k; e += 64) R[e] = 0.0;
+    __threadfence_block();
+    for (int j = 0; j < k; ++j) {
+        double* col = Q + (long long)j * rows;
+        double v[NV];
+#pragma unroll
+        for (int s = 0; s < NV; ++s) {
+            const int t = lane + 64 * s;
+            v[s] = t < rows ? col[t] : 0.0;
+        }
+        for (int pass = 0; pass < 2 && j > 0; ++pass) {
+            double q[NV], qn[NV];
+#pragma unroll
+            for (int s = 0; s < NV; ++s) {
+                const int t = lane + 64 * s;
+                q[s] = t < rows ? Q[t] : 0.0;
+                qn[s] = 0.0;
+            }
+            for (int i = 0; i < j; ++i) {
+                if (i + 1 < j) {
+                    const double* nx = Q + (long long)(i + 1) * rows;
+#pragma unroll
+                    for (int s = 0; s < NV; ++s) {
+                        const int t = lane + 64 * s;
+                        qn[s] = t < rows ? nx[t] : 0.0;
+                    }
+                }
+                double d = 0;
+#pragma unroll
+                for (int s = 0; s < NV; ++s) d += q[s] * v[s];
+                d = wsum(d);
+                if (lane == 0) R[i * k + j] += d;
+#pragma unroll
+                for (int s = 0; s < NV; ++s) {
+                    v[s] -= d * q[s];
+                    q[s] = qn[s];
+                }
+            }
+        }
+        double nv = 0;
+#pragma unroll
+        for (int s = 0; s < NV; ++s) nv += v[s] * v[s];
+        nv = sqrt(wsum(nv));
+        if (lane == 0) R[j * k + j] = nv;
+#pragma unroll
+        for (int s = 0; s < NV; ++s) {
+            const int t = lane + 64 * s;
+            if (nv > 0) v[s] /= nv;
+            if (t < rows) col[t] = v[s];
+        }
+    }
+    __threadfence_block();
+}
+
+// The same MGS2 on an NT-thread workgroup for long columns (rows <= NT NV):
+// column in registers across the block, sums over the block through LDS in a
+// fixed order (the same bits in every thread).  PF: load the next basis
+// column while the current one is applied (costs NV registers).
+template <int NV, int NT, bool PF>
+__device__ void dqr_mgs2_block(int rows, int k, double* Q, double* R) {
+    constexpr int NW = NT / 64;
+    __shared__ double red[2][NW];
+    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+    int flip = 0;
+    auto bsum = [&](double x) {
+        x = wsum(x);
+        if (lane == 0) red[flip][wv] = x;
+        __syncthreads();
+        double t = red[flip][0];
+#pragma unroll
+        for (int w = 1; w < NW; ++w) t += red[flip][w];
+        flip ^= 1;  // the other slot next time: no second barrier needed
+        return t;
+    };
+    for (int e = tid; e < k * k; e += NT) R[e] = 0.0;
+    __threadfence_block();
+    __syncthreads();
+    for (int j = 0; j < k; ++j) {
+        double* col = Q + (long long)j * rows;
+        double v[NV];
+#pragma unroll
+        for (int s = 0; s < NV; ++s) {
+            const int t = tid + NT * s;
+            v[s] = t < rows ? col[t] : 0.0;
+        }
+        for (int pass = 0; pass < 2 && j > 0; ++pass) {
+            double q[NV], qn[NV];
+#pragma unroll
+            for (int s = 0; s < NV; ++s) {
+                const int t = tid + NT * s;
+                q[s] = t < rows ? Q[t] : 0.0;
+                qn[s] = 0.0;
+            }
+            for (int i = 0; i < j; ++i) {
+                if (PF && i + 1 < j) {
+                    const double* nx = Q + (long long)(i + 1) * rows;
+#pragma unroll
+                    for (int s = 0; s < NV; ++s) {
+                        const int t = tid + NT * s;
+                        qn[s] = t < rows ? nx[t] : 0.0;
+                    }
+                }
+                double d = 0;
+#pragma unroll
+                for (int s = 0; s < NV; ++s) d += q[s] * v[s];
+                d = bsum(d);
+                if (tid == 0) R[i * k + j] += d;
+#pragma unroll
+                for (int s = 0; s < NV; ++s) v[s] -= d * q[s];
+                if (PF) {
+#pragma unroll
+                    for (int s = 0; s < NV; ++s) q[s] = qn[s];
+                } else if (i + 1 < j) {
+                    const double* nx = Q + (long long)(i + 1) * rows;
+#pragma unroll
+                    for (int s = 0; s < NV; ++s) {
+                        const int t = tid + NT * s;
+                        q[s] = t < rows ? nx[t] : 0.0;
+                    }
+                }
+            }
+        }
+        double nv = 0;
+#pragma unroll
+        for (int s = 0; s < NV; ++s) nv += v[s] * v[s];
+        nv = sqrt(bsum(nv));
+        if (tid == 0) R[j * k + j] = nv;
+#pragma unroll
+        for (int s = 0; s < NV; ++s) {
+            const int t = tid + NT * s;
+            if (nv > 0) v[s] /= nv;
+            if (t < rows) col[t] = v[s];
+        }
+        __threadfence_block();
+        __syncthreads();  // column j complete before other threads read it
+    }
+}
+
+// The same MGS2 on an NT-thread workgroup with the columns in memory (any
+// rows): each step reads the basis and current columns twice, spread over
+// NT / 64 waves.
+template <int NT>
+__device__ void dqr_mgs2_mem_block(int rows, int k, double* Q, double* R) {
+    constexpr int NW = NT / 64;
+    __shared__ double red[2][NW];
+    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+    int flip = 0;
+    auto bsum = [&](double x) {
+        x = wsum(x);
+        if (lane == 0) red[flip][wv] = x;
+        __syncthreads();
+        double t = red[flip][0];
+#pragma unroll
+        for (int w = 1; w < NW; ++w) t += red[flip][w];
+        flip ^= 1;
+        return t;
+    };
+    for (int e = tid; e < k * k; e += NT) R[e] = 0.0;
+    __threadfence_block();
+    __syncthreads();
+    for (int j = 0; j < k; ++j) {
+        double* v = Q + (long long)j * rows;
+        for (int pass = 0; pass < 2; ++pass)
+            for (int i = 0; i < j; ++i) {
+                const double* q = Q + (long long)i * rows;
+                double d = 0;
+                for (int t = tid; t < rows; t += NT) d += q[t] * v[t];
+                d = bsum(d);
+                if (tid == 0) R[i * k + j] += d;
+                for (int t = tid; t < rows; t += NT) v[t] -= d * q[t];
+            }
+        double nv = 0;
+        for (int t = tid; t < rows; t += NT) nv += v[t] * v[t];
+        nv = sqrt(bsum(nv));
+        if (tid == 0) R[j * k + j] = nv;
+        if (nv > 0)
+            for (int t = tid; t < rows; t += NT) v[t] /= nv;
+        __threadfence_block();
+        __syncthreads();
+    }
+}
+
+template <int NT>
+__global__ __launch_bounds__(NT) void aca_dev_qr_mem_block_kernel(const DAca* __restrict__ inst,
+                                                                  const int* __restrict__ jobs,
+                                                                  double* __restrict__ dbuf, double* __restrict__ ws,
+                                                                  const long long* __restrict__ wsoff) {
+    const int job = jobs[blockIdx.x], x = job >> 1, which = job & 1;
+    const DAca& s = inst[x];
+    const int k = s.k, rows = which ? s.n : s.m;
+    dqr_mgs2_mem_block<NT>(rows, k, dbuf + (which ? s.offV : s.offU), ws + wsoff[x] + (which ? (long long)k * k : 0));
+}
+
+// QR jobs on workgroups: one job (as below) per NT-thread block
+template <int NV, int NT, bool PF>
+__global__ __launch_bounds__(NT) void aca_dev_qr_block_kernel(const DAca* __restrict__ inst,
+                                                              const int* __restrict__ jobs,
+                                                              double* __restrict__ dbuf, double* __restrict__ ws,
+                                                              const long long* __restrict__ wsoff) {
+    const int job = jobs[blockIdx.x], x = job >> 1, which = job & 1;
+    const DAca& s = inst[x];
+    const int k = s.k, rows = which ? s.n : s.m;
+    dqr_mgs2_block<NV, NT, PF>(rows, k, dbuf + (which ? s.offV : s.offU),
+                           ws + wsoff[x] + (which ? (long long)k * k : 0));
+}
+
+// QR jobs: job = 2 x + w, w = 0 for U (m rows, R at wsoff[x]), 1 for V^T (n
+// rows, R at wsoff[x] + k^2); one wave per job
+template <int NV>
+__global__ __launch_bounds__(256) void aca_dev_qr_kernel(const DAca* __restrict__ inst,
+                                                         const int* __restrict__ jobs, int njobs,
+                                                         double* __restrict__ dbuf, double* __restrict__ ws,
+                                                         const long long* __restrict__ wsoff) {
+    const int w = __builtin_amdgcn_readfirstlane((int)(blockIdx.x * 4 + (threadIdx.x >> 6)));
+    if (w >= njobs) return;  // whole wave
+    const int job = jobs[w], x = job >> 1, which = job & 1;
+    const DAca& s = inst[x];
+    const int k = s.k, rows = which ? s.n : s.m;
+    double* Q = dbuf + (which ? s.offV : s.offU);
+    double* R = ws + wsoff[x] + (which ? (long long)k * k : 0);
+    dqr_mgs2_regs<NV>(rows, k, Q, R);
+}
+
+// U (offU, m x k) and V (offV, k x n) of instance x have become Qu and Qv^T in
+// place (aca_dev_qr_kernel) with Ru, Rv at ws + wsoff[x]; ws + wsoff[x] (4 k^2) receives the scaled singular vectors
+// Wh = W_r sqrt(sig) (column l at l*k) and Zh = Z_r sqrt(sig) (at k^2);
+// rank[x] = r.  k <= 64 (one column per lane in the SVD).
+__global__ __launch_bounds__(256) void aca_dev_recompress_kernel(const DAca* __restrict__ inst, long long ni,
+                                                                 double* __restrict__ ws,
+                                                                 const long long* __restrict__ wsoff, double eps,
+                                                                 int* __restrict__ rank) {
+    const long long x = __builtin_amdgcn_readfirstlane((int)(blockIdx.x * 4 + (threadIdx.x >> 6)));
+    if (x >= ni) return;  // whole wave
+    const int lane = threadIdx.x & 63;
+    const DAca& s = inst[x];
+    const int k = s.capped ? 0 : s.k;
+    if (k == 0) {
+        if (lane == 0) rank[x] = 0;
+        return;
+    }
+    double* Ru = ws + wsoff[x];
+    double* Rv = Ru + k * k;
+    double* A = Rv + k * k;   // columns of C = Ru Rv^T, rotated in place
+    double* Vj = A + k * k;   // accumulated rotations
+    for (int e = lane; e < k * k; e += 64) {
+        const int i = e / k, j = e % k;
+        double a = 0;
+        for (int t = 0; t < k; ++t) a += Ru[i * k + t] * Rv[j * k + t];
+        A[j * k + i] = a;
+        Vj[e] = (i == j) ? 1.0 : 0.0;
+    }
+    __threadfence_block();
+    // one-sided Jacobi, cyclic order; lane t owns row t of A and Vj
+    const bool own = lane < k;
+    for (int sweep = 0; sweep < 60; ++sweep) {
+        bool rotated = false;
+        for (int p = 0; p < k - 1; ++p)
+            for (int q = p + 1; q < k; ++q) {
+                const double u = own ? A[p * k + lane] : 0.0, w = own ? A[q * k + lane] : 0.0;
+                const double al = wsum(u * u), be = wsum(w * w), ga = wsum(u * w);
+                if (ga == 0.0 || fabs(ga) <= 1e-15 * sqrt(al * be)) continue;
+                rotated = true;
+                const double zeta = (be - al) / (2.0 * ga);
+                const double t = (zeta >= 0 ? 1.0 : -1.0) / (fabs(zeta) + sqrt(1.0 + zeta * zeta));
+                const double c = 1.0 / sqrt(1.0 + t * t), sn = c * t;
+                if (own) {
+                    A[p * k + lane] = c * u - sn * w;
+                    A[q * k + lane] = sn * u + c * w;
+                    const double vu = Vj[p * k + lane], vw = Vj[q * k + lane];
+                    Vj[p * k + lane] = c * vu - sn * vw;
+                    Vj[q * k + lane] = sn * vu + c * vw;
+                }
+            }
+        if (!rotated) break;
+    }
+    __threadfence_block();
+    // column norms; lane j holds column j's, its stable descending position
+    double nrm = 0;
+    if (own) {
+        double a = 0;
+        for (int t = 0; t < k; ++t) a += A[lane * k + t] * A[lane * k + t];
+        nrm = sqrt(a);
+    }
+    int pos = 0;
+    for (int i = 0; i < k; ++i) {
+        const double o = __shfl(nrm, i, 64);
+        pos += (o > nrm) || (o == nrm && i < lane);
+    }
+    // truncation in descending order, as the host loop sums
+    int r = 0;
+    {
+        double tot = 0;
+        for (int l = 0; l < k; ++l) {
+            const int src = __ffsll((long long)__ballot(own && pos == l)) - 1;
+            const double sg = __shfl(nrm, src, 64);
+            tot += sg * sg;
+        }
+        if (tot > 0) {
+            double tail = tot;
+            while (r < k && tail > eps * eps * tot) {
+                const int src = __ffsll((long long)__ballot(own && pos == r)) - 1;
+                const double sg = __shfl(nrm, src, 64);
+                tail -= sg * sg;
+                ++r;
+            }
+        }
+    }
+    // Wh, Zh over Ru, Rv (no longer needed)
+    double* Wh = Ru;
+    double* Zh = Rv;
+    for (int l = 0; l < r; ++l) {
+        const int src = __ffsll((long long)__ballot(own && pos == l)) - 1;
+        if (src < 0) {  // positions are a permutation unless a norm is NaN
+            r = l;
+            break;
+        }
+        const double sg = __shfl(nrm, src, 64);
+        const double h = sqrt(sg);
+        if (own) {
+            Wh[l * k + lane] = (sg > 0 ? A[src * k + lane] / sg : 0.0) * h;
+            Zh[l * k + lane] = Vj[src * k + lane] * h;
+        }
+    }
+    if (lane == 0) rank[x] = r;
+}
+
+// final factors in the fetch layout: U2 (m x r, column-major) at dU + uoff[x],
+// V2 (r x n, row-major) at dV + voff[x]; U2 = Qu Wh, V2 = Zh^T Qv^T, or with
+// ident the first k terms as they are (no recompression)
+__global__ __launch_bounds__(256) void aca_dev_form_kernel(const DAca* __restrict__ inst,
+                                                           const double* __restrict__ dbuf,
+                                                           const double* __restrict__ ws,
+                                                           const long long* __restrict__ wsoff,
+                                                           const int* __restrict__ rank,
+                                                           const long long* __restrict__ uoff,
+                                                           const long long* __restrict__ voff, int ident,
+                                                           double* __restrict__ dU, double* __restrict__ dV) {
+    const long long x = blockIdx.x;
+    const DAca& s = inst[x];
+    if (s.capped) return;
+    const int k = s.k, m = s.m, n = s.n;
+    const int r = ident ? k : rank[x];
+    if (r == 0) return;
+    const double* Qu = dbuf + s.offU;
+    const double* Qv = dbuf + s.offV;
+    double* U2 = dU + uoff[x];
+    double* V2 = dV + voff[x];
+    if (ident) {
+        for (long long t = threadIdx.x; t < (long long)k * m; t += blockDim.x) U2[t] = Qu[t];
+        for (long long t = threadIdx.x; t < (long long)k * n; t += blockDim.x) V2[t] = Qv[t];
+        return;
+    }
+    const double* Wh = ws + wsoff[x];
+    const double* Zh = Wh + k * k;
+    for (int l = 0; l < r; ++l) {
+        for (int t = threadIdx.x; t < m; t += blockDim.x) {
+            double u = 0;
+            for (int i = 0; i < k; ++i) {
+                const double w = Wh[l * k + i];
+                if (w != 0.0) u += w * Qu[(long long)i * m + t];
+            }
+            U2[(long long)l * m + t] = u;
+        }
+        for (int t = threadIdx.x; t < n; t += blockDim.x) {
+            double v = 0;
+            for (int i = 0; i < k; ++i) {
+                const double z = Zh[l * k + i];
+                if (z != 0.0) v += z * Qv[(long long)i * n + t];
+            }
+            V2[(long long)l * n + t] = v;
+        }
+    }
+}
 
 }  // namespace pq3d
 
@@ -732,11 +1725,8 @@ int pq3d_gf_segments(void* h, const int* rcv_idx, long long nr_idx, const int* s
 // eps: relative tolerance; recompress_eps > 0: recompress every factor pair
 // to that tolerance afterwards; max_rank <= 0: min(m, n).  ranks (nb x 6) receive
 // the ranks; *result holds the factors until pq3d_aca_free.
-int pq3d_aca_run(void* h, long long nb, const long long* row_off, const int* row_idx, const long long* col_off,
-                 const int* col_idx, double eps, double recompress_eps, int max_rank, unsigned long long seed,
-                 long long* ranks, void** result) {
-    clear_error();
-    GFSource* g = (GFSource*)h;
+static int aca_check(GFSource* g, long long nb, const long long* row_off, const int* row_idx,
+                     const long long* col_off, const int* col_idx, double eps) {
     PQ_CHECK(g, "pq3d_aca_run: null source");
     PQ_CHECK(eps > 0 && eps < 1, "pq3d_aca_run: eps must be in (0, 1)");
     const long long nr = row_off[nb], nc = col_off[nb];
@@ -746,123 +1736,37 @@ int pq3d_aca_run(void* h, long long nb, const long long* row_off, const int* row
         PQ_CHECK(row_idx[i] >= 0 && row_idx[i] < g->nrcv, "pq3d_aca_run: row id out of range");
     for (long long i = 0; i < nc; ++i)
         PQ_CHECK(col_idx[i] >= 0 && col_idx[i] < g->nsrc, "pq3d_aca_run: column id out of range");
-    auto t_start = std::chrono::steady_clock::now();
-    AcaResult* res = new AcaResult();
-    res->nb = nb;
-    int* d_idx = nullptr;
-    hipError_t e = hipMalloc((void**)&d_idx, (size_t)(nr + nc + 1) * sizeof(int));
-    if (e == hipSuccess) e = hipMemcpy(d_idx, row_idx, nr * sizeof(int), hipMemcpyHostToDevice);
-    if (e == hipSuccess) e = hipMemcpy(d_idx + nr, col_idx, nc * sizeof(int), hipMemcpyHostToDevice);
+    return 0;
+}
+
+static int upload_ids(const long long* row_off, const int* row_idx, const long long* col_off, const int* col_idx,
+                      long long nb, int** d_idx) {
+    const long long nr = row_off[nb], nc = col_off[nb];
+    *d_idx = nullptr;
+    PQ_HIP(hipMalloc((void**)d_idx, (size_t)(nr + nc + 1) * sizeof(int)));
+    hipError_t e = hipMemcpy(*d_idx, row_idx, nr * sizeof(int), hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(*d_idx + nr, col_idx, nc * sizeof(int), hipMemcpyHostToDevice);
     if (e != hipSuccess) {
-        (void)hipFree(d_idx);
-        delete res;
+        (void)hipFree(*d_idx);
+        *d_idx = nullptr;
         set_error("pq3d_aca_run: %s", hipGetErrorString(e));
         return -1;
     }
-    res->inst.resize((size_t)nb * 6);
-    for (long long b = 0; b < nb; ++b) {
-        const int m = (int)(row_off[b + 1] - row_off[b]), n = (int)(col_off[b + 1] - col_off[b]);
-        uint64_t bs = seed ^ (0x51ED270B27A5ull * (uint64_t)(b + 1));
-        const int i0 = (int)(splitmix(bs) % (uint64_t)m), j0 = (int)(splitmix(bs) % (uint64_t)n);
-        for (int c = 0; c < 6; ++c) {
-            Inst& s = res->inst[(size_t)b * 6 + c];
-            s.b = (int)b;
-            s.c = c;
-            s.m = m;
-            s.n = n;
-            s.kmax = std::min(m, n);
-            if (max_rank > 0) s.kmax = std::min(s.kmax, max_rank);
-            s.ur.assign(m, 0);
-            s.uc.assign(n, 0);
-            s.Rr.assign(n, 0.0);
-            s.Rc.assign(m, 0.0);
-            s.rng = bs ^ (0x9E37ull * (uint64_t)(c + 1));
-            // all six matrices of a block start from the same reference row/column
-            s.iref = i0;
-            s.jref = j0;
-            push_req(s, 0, i0);
-            push_req(s, 1, j0);
-            s.state = ST_REFS;
-        }
-    }
-    std::vector<Req> reqs;
-    std::vector<SegDesc> segs;
-    std::vector<double> out;
-    std::unordered_map<uint64_t, int> key;
-    double gpu_ms = 0, pairs_total = 0;
-    long long sweeps = 0;
-    int rc = 0;
-    const long long max_sweeps = 4LL * (long long)(max_rank > 0 ? max_rank : 4096) + 64;
+    return 0;
+}
+
+// recompression, ranks and statistics shared by both entry points
+static void aca_finish(AcaResult* res, double recompress_eps, long long* ranks, const AcaCounters& cnt,
+                       std::chrono::steady_clock::time_point t_start) {
     using clk = std::chrono::steady_clock;
-    auto secs = [](clk::time_point a, clk::time_point b) { return std::chrono::duration<double>(b - a).count(); };
-    while (true) {
-        const auto t0 = clk::now();
-        reqs.clear();
-        key.clear();
-        for (auto& s : res->inst) {
-            if (s.state == ST_DONE) continue;
-            for (int t = 0; t < s.nq; ++t) {
-                const uint64_t kk = ((uint64_t)s.b << 33) | ((uint64_t)s.qt[t] << 32) | (uint32_t)s.qi[t];
-                auto it = key.find(kk);
-                int q;
-                if (it == key.end()) {
-                    q = (int)reqs.size();
-                    key.emplace(kk, q);
-                    reqs.push_back(Req{s.b, s.qt[t], s.qi[t], 0});
-                } else {
-                    q = it->second;
-                }
-                reqs[q].mask |= (s.scale < 0 ? 0x3f : (1 << s.c));
-                s.qn[t] = q;
-            }
-        }
-        if (reqs.empty()) break;
-        if (++sweeps > max_sweeps) {
-            set_error("pq3d_aca_run: no convergence after %lld sweeps", sweeps);
-            rc = -3;
-            break;
-        }
-        segs.resize(reqs.size());
-        long long pairs = 0, outs = 0;
-        for (size_t q = 0; q < reqs.size(); ++q) {
-            const Req& r = reqs[q];
-            const long long ro = row_off[r.b], co = col_off[r.b];
-            const int m = (int)(row_off[r.b + 1] - ro), n = (int)(col_off[r.b + 1] - co);
-            SegDesc d;
-            if (r.type == 0) d = SegDesc{pairs, outs, (int)(ro + r.idx), 1, (int)co, n, r.mask, 0};
-            else d = SegDesc{pairs, outs, (int)ro, m, (int)(co + r.idx), 1, r.mask, 0};
-            segs[q] = d;
-            pairs += (long long)d.m * d.n;
-            outs += (long long)popc(r.mask) * d.m * d.n;
-        }
-        if ((long long)out.size() < outs) out.resize((size_t)outs);
-        const auto t1 = clk::now();
-        rc = run_segments(g, d_idx, d_idx + nr, segs, pairs, outs, out.data(), &gpu_ms);
-        if (rc) break;
-        const auto t2 = clk::now();
-        pairs_total += (double)pairs;
-        Results R{&reqs, &segs, out.data()};
-        parallel_for((long long)res->inst.size(), [&](long long x) {
-            Inst& s = res->inst[(size_t)x];
-            if (s.state != ST_DONE && s.nq) consume(s, R, eps);
-        });
-        const auto t3 = clk::now();
-        res->phase_s[0] += secs(t0, t1);
-        res->phase_s[1] += secs(t1, t2);
-        res->phase_s[2] += secs(t2, t3);
-    }
-    (void)hipFree(d_idx);
-    if (rc) {
-        delete res;
-        return rc;
-    }
     const auto t_rc = clk::now();
     if (recompress_eps > 0)
         parallel_for((long long)res->inst.size(), [&](long long x) {
             Inst& s = res->inst[(size_t)x];
             s.k = recompress_factors(s.m, s.n, s.k, s.U, s.V, recompress_eps);
         });
-    res->phase_s[3] = secs(t_rc, clk::now());
+    for (int k = 0; k < 3; ++k) res->phase_s[k] = cnt.phase_s[k];
+    res->phase_s[3] = std::chrono::duration<double>(clk::now() - t_rc).count();
     int maxk = 0;
     long long unconv = 0;
     for (auto& s : res->inst) {
@@ -870,16 +1774,357 @@ int pq3d_aca_run(void* h, long long nb, const long long* row_off, const int* row
         maxk = std::max(maxk, s.k);
         if (!s.converged) ++unconv;
     }
-    const double total_s =
-        std::chrono::duration<double>(std::chrono::steady_clock::now() - t_start).count();
-    res->stats[0] = (double)sweeps;
-    res->stats[1] = (double)sweeps;
-    res->stats[2] = pairs_total;
-    res->stats[3] = gpu_ms * 1e-3;
-    res->stats[4] = total_s - gpu_ms * 1e-3;
+    const double total_s = std::chrono::duration<double>(clk::now() - t_start).count();
+    res->stats[0] = (double)cnt.sweeps;
+    res->stats[1] = (double)cnt.sweeps;
+    res->stats[2] = cnt.pairs;
+    res->stats[3] = cnt.gpu_ms * 1e-3;
+    res->stats[4] = total_s - cnt.gpu_ms * 1e-3;
     res->stats[5] = maxk;
     res->stats[6] = (double)unconv;
     res->stats[7] = total_s;
+}
+
+int pq3d_aca_run(void* h, long long nb, const long long* row_off, const int* row_idx, const long long* col_off,
+                 const int* col_idx, double eps, double recompress_eps, int max_rank, unsigned long long seed,
+                 long long* ranks, void** result) {
+    clear_error();
+    GFSource* g = (GFSource*)h;
+    if (int rc = aca_check(g, nb, row_off, row_idx, col_off, col_idx, eps)) return rc;
+    const auto t_start = std::chrono::steady_clock::now();
+    int* d_idx = nullptr;
+    if (upload_ids(row_off, row_idx, col_off, col_idx, nb, &d_idx)) return -1;
+    AcaResult* res = new AcaResult();
+    res->nb = nb;
+    aca_init_instances(nb, row_off, col_off, nullptr, max_rank, seed, res->inst);
+    AcaCounters cnt;
+    const int rc = aca_host_loop(g, d_idx, row_off, col_off, nb, eps, max_rank, res->inst, cnt);
+    (void)hipFree(d_idx);
+    if (rc) {
+        delete res;
+        return rc;
+    }
+    aca_finish(res, recompress_eps, ranks, cnt, t_start);
+    *result = res;
+    return 0;
+}
+
+// The same ACA with the pivoting on the device (one wave per block and
+// matrix; no per-sweep transfer of values).  Factors of at most kcap terms
+// live on the device (kcap <= 0: 48, at most 64); an instance that needs more is redone
+// by the host loop from its own seed.  Results, ranks and statistics as
+// pq3d_aca_run (pq3d_aca_fetch/stats/phases/free apply).
+int pq3d_aca_run_device(void* h, long long nb, const long long* row_off, const int* row_idx,
+                        const long long* col_off, const int* col_idx, double eps, double recompress_eps,
+                        int max_rank, unsigned long long seed, int kcap, long long* ranks, void** result) {
+    clear_error();
+    GFSource* g = (GFSource*)h;
+    if (int rc = aca_check(g, nb, row_off, row_idx, col_off, col_idx, eps)) return rc;
+    if (kcap <= 0) kcap = 48;
+    PQ_CHECK(kcap <= 64, "pq3d_aca_run_device: kcap %d above 64 (one SVD column per lane)", kcap);
+    using clk = std::chrono::steady_clock;
+    const auto t_start = clk::now();
+    AcaResult* res = new AcaResult();
+    res->nb = nb;
+    aca_init_instances(nb, row_off, col_off, nullptr, max_rank, seed, res->inst);
+    const long long ni = (long long)res->inst.size();
+    // device instances and their buffer offsets
+    std::vector<DAca> dh((size_t)ni);
+    long long dtot = 0, ftot = 0;
+    for (long long x = 0; x < ni; ++x) {
+        const Inst& s = res->inst[(size_t)x];
+        DAca d;
+        memset(&d, 0, sizeof(d));
+        d.b = s.b;
+        d.c = s.c;
+        d.m = s.m;
+        d.n = s.n;
+        d.kmax = s.kmax;
+        d.kcap = std::min(s.kmax, kcap);
+        d.state = ST_REFS;
+        d.iref = s.iref;
+        d.jref = s.jref;
+        d.istar = d.jstar = -1;
+        d.nq = s.nq;
+        d.need_rr = d.need_rc = 1;
+        d.first = 1;
+        for (int t = 0; t < 2; ++t) {
+            d.qt[t] = s.qt[t];
+            d.qi[t] = s.qi[t];
+        }
+        d.rng = s.rng;
+        d.offU = dtot;
+        dtot += (long long)d.kcap * s.m;
+        d.offV = dtot;
+        dtot += (long long)d.kcap * s.n;
+        d.offM = dtot;
+        dtot += 3LL * s.m;
+        d.offN = dtot;
+        dtot += 3LL * s.n;
+        d.offFm = ftot;
+        ftot += s.m;
+        d.offFn = ftot;
+        ftot += s.n;
+        dh[(size_t)x] = d;
+    }
+    std::vector<long long> cnt0((size_t)ni);
+    for (long long x = 0; x < ni; ++x) cnt0[(size_t)x] = (long long)dh[(size_t)x].m + dh[(size_t)x].n;
+    int* d_idx = nullptr;
+    DAca* d_inst = nullptr;
+    double* d_buf = nullptr;
+    unsigned char* d_flag = nullptr;
+    long long *d_cnt = nullptr, *d_off = nullptr, *d_offs = nullptr;
+    AcaCounters cnt;
+    int rc = upload_ids(row_off, row_idx, col_off, col_idx, nb, &d_idx);
+    hipError_t e = hipSuccess;
+// each step runs only while every earlier one succeeded
+#define ACA_TRY(x)                        \
+    do {                                  \
+        if (e == hipSuccess) e = (x);     \
+    } while (0)
+    if (!rc) {
+        ACA_TRY(hipMalloc((void**)&d_inst, (size_t)ni * sizeof(DAca)));
+        ACA_TRY(hipMalloc((void**)&d_buf, (size_t)std::max(dtot, 1LL) * sizeof(double)));
+        ACA_TRY(hipMalloc((void**)&d_flag, (size_t)std::max(ftot, 1LL)));
+        ACA_TRY(hipMalloc((void**)&d_cnt, (size_t)ni * sizeof(long long)));
+        ACA_TRY(hipMalloc((void**)&d_off, (size_t)(ni + 1) * sizeof(long long)));
+        ACA_TRY(hipMalloc((void**)&d_offs, (size_t)(2 * nb + 2) * sizeof(long long)));
+        ACA_TRY(hipMemcpy(d_inst, dh.data(), (size_t)ni * sizeof(DAca), hipMemcpyHostToDevice));
+        ACA_TRY(hipMemset(d_buf, 0, (size_t)std::max(dtot, 1LL) * sizeof(double)));
+        ACA_TRY(hipMemset(d_flag, 0, (size_t)std::max(ftot, 1LL)));
+        ACA_TRY(hipMemcpy(d_cnt, cnt0.data(), (size_t)ni * sizeof(long long), hipMemcpyHostToDevice));
+        ACA_TRY(hipMemcpy(d_offs, row_off, (size_t)(nb + 1) * sizeof(long long), hipMemcpyHostToDevice));
+        ACA_TRY(hipMemcpy(d_offs + nb + 1, col_off, (size_t)(nb + 1) * sizeof(long long), hipMemcpyHostToDevice));
+    }
+    const long long nr = row_off[nb];
+    const long long max_sweeps = 4LL * (long long)(max_rank > 0 ? max_rank : 4096) + 64;
+    hipEvent_t ea = nullptr, eb = nullptr;
+    if (!rc && e == hipSuccess) {
+        ACA_TRY(hipEventCreate(&ea));
+        ACA_TRY(hipEventCreate(&eb));
+    }
+    const auto t_swept0 = clk::now();
+    res->dev[1] = std::chrono::duration<double>(t_swept0 - t_start).count();
+    bool first_sweep = true;
+    while (!rc && e == hipSuccess) {
+        const auto t0 = clk::now();
+        if (!first_sweep)
+            hipLaunchKernelGGL(aca_dev_logic_kernel, dim3((unsigned)((ni + 3) / 4)), dim3(256), 0, 0, d_inst, ni,
+                               d_buf, d_flag, eps, d_cnt);
+        first_sweep = false;
+        hipLaunchKernelGGL(aca_dev_scan_kernel, dim3(1), dim3(1024), 0, 0, d_cnt, ni, d_off);
+        long long total = 0;
+        ACA_TRY(hipGetLastError());
+        ACA_TRY(hipMemcpy(&total, d_off + ni, sizeof(long long), hipMemcpyDeviceToHost));
+        if (e != hipSuccess) break;
+        const auto t1 = clk::now();
+        if (total == 0) break;
+        if (++cnt.sweeps > max_sweeps) {
+            set_error("pq3d_aca_run_device: no convergence after %lld sweeps", cnt.sweeps);
+            rc = -3;
+            break;
+        }
+        const long long nblk = (total + 255) / 256;
+        if (nblk >= (1LL << 31)) {
+            set_error("pq3d_aca_run_device: sweep too large (%lld pairs)", total);
+            rc = -2;
+            break;
+        }
+        ACA_TRY(hipEventRecord(ea, 0));
+        hipLaunchKernelGGL(aca_dev_eval_kernel, dim3((unsigned)nblk), dim3(256), 0, 0, g->geo, g->xg, g->ev, d_idx,
+                           d_idx + nr, d_offs, d_offs + nb + 1, d_inst, d_off, ni, g->mu, g->lam, g->nu,
+                           g->halfspace, d_buf);
+        ACA_TRY(hipGetLastError());
+        ACA_TRY(hipEventRecord(eb, 0));
+        ACA_TRY(hipEventSynchronize(eb));
+        float ms = 0;
+        ACA_TRY(hipEventElapsedTime(&ms, ea, eb));
+        cnt.gpu_ms += ms;
+        cnt.pairs += (double)total;
+        const auto t2 = clk::now();
+        cnt.phase_s[2] += std::chrono::duration<double>(t1 - t0).count();  // logic + scan (device)
+        cnt.phase_s[1] += std::chrono::duration<double>(t2 - t1).count();  // evaluation
+    }
+    const auto t_swept1 = clk::now();
+    res->dev[2] = std::chrono::duration<double>(t_swept1 - t_swept0).count();
+    std::vector<long long> redo;
+    if (!rc && e == hipSuccess) {
+        ACA_TRY(hipMemcpy(dh.data(), d_inst, (size_t)ni * sizeof(DAca), hipMemcpyDeviceToHost));
+        if (e == hipSuccess)
+            for (long long x = 0; x < ni; ++x)
+                if (dh[(size_t)x].capped && (redo.empty() || redo.back() != dh[(size_t)x].b))
+                    redo.push_back(dh[(size_t)x].b);
+    }
+    // instances that reached kcap: the host loop redoes their blocks and
+    // recompresses them on the host
+    double host_rc_s = 0;
+    if (!rc && e == hipSuccess && !redo.empty()) {
+        std::vector<long long> sro(1, 0), sco(1, 0);
+        std::vector<int> sri, sci;
+        for (long long b : redo) {
+            for (long long i = row_off[b]; i < row_off[b + 1]; ++i) sri.push_back(row_idx[i]);
+            for (long long i = col_off[b]; i < col_off[b + 1]; ++i) sci.push_back(col_idx[i]);
+            sro.push_back((long long)sri.size());
+            sco.push_back((long long)sci.size());
+        }
+        const long long nbs = (long long)redo.size();
+        std::vector<Inst> sub;
+        aca_init_instances(nbs, sro.data(), sco.data(), redo.data(), max_rank, seed, sub);
+        int* d_sidx = nullptr;
+        AcaCounters hc;  // the host loop's own sweep limit
+        rc = upload_ids(sro.data(), sri.data(), sco.data(), sci.data(), nbs, &d_sidx);
+        if (!rc) rc = aca_host_loop(g, d_sidx, sro.data(), sco.data(), nbs, eps, max_rank, sub, hc);
+        if (d_sidx) (void)hipFree(d_sidx);
+        cnt.sweeps += hc.sweeps;
+        cnt.pairs += hc.pairs;
+        cnt.gpu_ms += hc.gpu_ms;
+        for (int k = 0; k < 3; ++k) cnt.phase_s[k] += hc.phase_s[k];
+        if (!rc) {
+            std::vector<long long> mine;
+            for (long long q = 0; q < nbs; ++q)
+                for (int c = 0; c < 6; ++c) {
+                    const long long x = redo[(size_t)q] * 6 + c;
+                    if (!dh[(size_t)x].capped) continue;
+                    Inst& s = res->inst[(size_t)x];
+                    Inst& t = sub[(size_t)q * 6 + c];
+                    s.k = t.k;
+                    s.converged = t.converged;
+                    s.state = ST_DONE;
+                    s.U.swap(t.U);
+                    s.V.swap(t.V);
+                    mine.push_back(x);
+                }
+            const auto t_h = clk::now();
+            if (recompress_eps > 0)
+                parallel_for((long long)mine.size(), [&](long long y) {
+                    Inst& s = res->inst[(size_t)mine[(size_t)y]];
+                    s.k = recompress_factors(s.m, s.n, s.k, s.U, s.V, recompress_eps);
+                });
+            host_rc_s = std::chrono::duration<double>(clk::now() - t_h).count();
+        }
+        res->dev[0] = (double)nbs;
+    }
+    // recompression of the device instances on the device, then every device
+    // instance's final factors in the fetch layout (block-major, then matrix)
+    const auto t_rc0 = clk::now();
+    double* d_ws = nullptr;
+    long long *d_wsoff = nullptr, *d_uoff = nullptr, *d_voff = nullptr;
+    int *d_rank = nullptr, *d_jobs = nullptr;
+    if (!rc && e == hipSuccess) {
+        const bool recompress = recompress_eps > 0;
+        std::vector<long long> wsoff((size_t)ni, 0);
+        long long wtot = 0;
+        for (long long x = 0; x < ni; ++x) {
+            const long long k = dh[(size_t)x].capped ? 0 : dh[(size_t)x].k;
+            wsoff[(size_t)x] = wtot;
+            wtot += 4 * k * k;
+        }
+        std::vector<int> rk((size_t)ni, 0);
+        if (recompress) {
+            ACA_TRY(hipMalloc((void**)&d_ws, (size_t)std::max(wtot, 1LL) * sizeof(double)));
+            ACA_TRY(hipMalloc((void**)&d_wsoff, (size_t)ni * sizeof(long long)));
+            ACA_TRY(hipMalloc((void**)&d_rank, (size_t)ni * sizeof(int)));
+            ACA_TRY(hipMemcpy(d_wsoff, wsoff.data(), (size_t)ni * sizeof(long long), hipMemcpyHostToDevice));
+            // QR jobs grouped by rows: one wave with 64 NV rows in registers
+            // (NV 1..16), one workgroup with 256 NV rows (NV 8, 16, 32), one
+            // 1024-thread workgroup with the columns in memory beyond that
+            const int NB = 9;
+            std::vector<int> jobs[NB];
+            for (long long x = 0; x < ni; ++x) {
+                const DAca& d = dh[(size_t)x];
+                if (d.capped || d.k == 0) continue;
+                for (int w = 0; w < 2; ++w) {
+                    const int rows = w ? d.n : d.m;
+                    const int bkt = rows <= 64 ? 0 : rows <= 128 ? 1 : rows <= 256 ? 2 : rows <= 512 ? 3
+                                  : rows <= 1024 ? 4 : rows <= 2048 ? 5 : rows <= 4096 ? 6 : rows <= 8192 ? 7 : 8;
+                    jobs[bkt].push_back((int)(2 * x + w));
+                }
+            }
+            std::vector<int> all;
+            size_t jo[NB + 1] = {0};
+            for (int b = 0; b < NB; ++b) {
+                all.insert(all.end(), jobs[b].begin(), jobs[b].end());
+                jo[b + 1] = all.size();
+            }
+            ACA_TRY(hipMalloc((void**)&d_jobs, std::max(all.size(), (size_t)1) * sizeof(int)));
+            ACA_TRY(hipMemcpy(d_jobs, all.data(), all.size() * sizeof(int), hipMemcpyHostToDevice));
+            for (int b = 0; b < NB && e == hipSuccess; ++b) {
+                const int nj = (int)(jo[b + 1] - jo[b]);
+                if (!nj) continue;
+                const dim3 grid((unsigned)((nj + 3) / 4)), gridb((unsigned)nj), blk(256), blk1024(1024);
+                const int* jp = d_jobs + jo[b];
+                switch (b) {
+                    case 0: hipLaunchKernelGGL(aca_dev_qr_kernel<1>, grid, blk, 0, 0, d_inst, jp, nj, d_buf, d_ws, d_wsoff); break;
+                    case 1: hipLaunchKernelGGL(aca_dev_qr_kernel<2>, grid, blk, 0, 0, d_inst, jp, nj, d_buf, d_ws, d_wsoff); break;
+                    case 2: hipLaunchKernelGGL(aca_dev_qr_kernel<4>, grid, blk, 0, 0, d_inst, jp, nj, d_buf, d_ws, d_wsoff); break;
+                    case 3: hipLaunchKernelGGL(aca_dev_qr_kernel<8>, grid, blk, 0, 0, d_inst, jp, nj, d_buf, d_ws, d_wsoff); break;
+                    case 4: hipLaunchKernelGGL(aca_dev_qr_kernel<16>, grid, blk, 0, 0, d_inst, jp, nj, d_buf, d_ws, d_wsoff); break;
+                    case 5: hipLaunchKernelGGL((aca_dev_qr_block_kernel<8, 256, true>), gridb, blk, 0, 0, d_inst, jp, d_buf, d_ws, d_wsoff); break;
+                    case 6: hipLaunchKernelGGL((aca_dev_qr_block_kernel<16, 256, true>), gridb, blk, 0, 0, d_inst, jp, d_buf, d_ws, d_wsoff); break;
+                    case 7: hipLaunchKernelGGL((aca_dev_qr_block_kernel<32, 256, true>), gridb, blk, 0, 0, d_inst, jp, d_buf, d_ws, d_wsoff); break;
+                    default: hipLaunchKernelGGL(aca_dev_qr_mem_block_kernel<1024>, gridb, blk1024, 0, 0, d_inst, jp, d_buf, d_ws, d_wsoff); break;
+                }
+                ACA_TRY(hipGetLastError());
+            }
+            if (e == hipSuccess) {
+                hipLaunchKernelGGL(aca_dev_recompress_kernel, dim3((unsigned)((ni + 3) / 4)), dim3(256), 0, 0,
+                                   d_inst, ni, d_ws, d_wsoff, recompress_eps, d_rank);
+                ACA_TRY(hipGetLastError());
+            }
+            ACA_TRY(hipMemcpy(rk.data(), d_rank, (size_t)ni * sizeof(int), hipMemcpyDeviceToHost));
+        } else {
+            for (long long x = 0; x < ni; ++x) rk[(size_t)x] = dh[(size_t)x].capped ? 0 : dh[(size_t)x].k;
+        }
+        res->host_inst.assign((size_t)ni, 0);
+        std::vector<long long> uoff((size_t)ni), voff((size_t)ni);
+        long long uo = 0, vo = 0;
+        for (long long x = 0; x < ni && e == hipSuccess; ++x) {
+            const DAca& d = dh[(size_t)x];
+            Inst& s = res->inst[(size_t)x];
+            if (d.capped) {
+                res->host_inst[(size_t)x] = 1;
+            } else {
+                s.k = rk[(size_t)x];
+                s.converged = d.converged != 0;
+                s.state = ST_DONE;
+            }
+            uoff[(size_t)x] = uo;
+            voff[(size_t)x] = vo;
+            uo += (long long)s.k * s.m;
+            vo += (long long)s.k * s.n;
+        }
+        res->usz = uo;
+        res->vsz = vo;
+        ACA_TRY(hipMalloc((void**)&res->dU, (size_t)std::max(uo, 1LL) * sizeof(double)));
+        ACA_TRY(hipMalloc((void**)&res->dV, (size_t)std::max(vo, 1LL) * sizeof(double)));
+        ACA_TRY(hipMalloc((void**)&d_uoff, (size_t)ni * sizeof(long long)));
+        ACA_TRY(hipMalloc((void**)&d_voff, (size_t)ni * sizeof(long long)));
+        ACA_TRY(hipMemcpy(d_uoff, uoff.data(), (size_t)ni * sizeof(long long), hipMemcpyHostToDevice));
+        ACA_TRY(hipMemcpy(d_voff, voff.data(), (size_t)ni * sizeof(long long), hipMemcpyHostToDevice));
+        if (e == hipSuccess) {
+            hipLaunchKernelGGL(aca_dev_form_kernel, dim3((unsigned)ni), dim3(256), 0, 0, d_inst, d_buf, d_ws,
+                               d_wsoff, d_rank, d_uoff, d_voff, recompress ? 0 : 1, res->dU, res->dV);
+            ACA_TRY(hipGetLastError());
+        }
+        ACA_TRY(hipDeviceSynchronize());
+    }
+    const double dev_rc_s = std::chrono::duration<double>(clk::now() - t_rc0).count();
+    if (eb) (void)hipEventDestroy(eb);
+    if (ea) (void)hipEventDestroy(ea);
+    void* bufs[] = {d_idx, d_inst, d_buf, d_flag, d_cnt, d_off, d_offs, d_ws, d_wsoff, d_rank, d_uoff, d_voff, d_jobs};
+    for (void* p : bufs)
+        if (p) (void)hipFree(p);
+#undef ACA_TRY
+    if (!rc && e != hipSuccess) {
+        set_error("pq3d_aca_run_device: %s", hipGetErrorString(e));
+        rc = -1;
+    }
+    if (rc) {
+        delete res;
+        return rc;
+    }
+    aca_finish(res, 0.0, ranks, cnt, t_start);  // recompressed above
+    res->phase_s[3] = host_rc_s + dev_rc_s;
     *result = res;
     return 0;
 }
@@ -904,6 +2149,17 @@ int pq3d_aca_phases(void* r, double* secs4) {
     return 0;
 }
 
+// pq3d_aca_run_device result: blocks the host loop redid because one of their
+// matrices needed more than kcap terms, then wall seconds of set-up, sweeps and
+// factor readback by pq3d_aca_fetch (all 0 for pq3d_aca_run).
+int pq3d_aca_device_info(void* r, double* info4) {
+    clear_error();
+    AcaResult* res = (AcaResult*)r;
+    PQ_CHECK(res && info4, "pq3d_aca_device_info: null argument");
+    for (int k = 0; k < 4; ++k) info4[k] = res->dev[k];
+    return 0;
+}
+
 // Factors, block-major then matrix A1s..Bd: U (m x k, column-major) into U,
 // V (k x n, row-major) into V, back to back.
 int pq3d_aca_fetch(void* r, double* U, double* V) {
@@ -911,6 +2167,23 @@ int pq3d_aca_fetch(void* r, double* U, double* V) {
     AcaResult* res = (AcaResult*)r;
     PQ_CHECK(res, "pq3d_aca_fetch: null result");
     size_t uo = 0, vo = 0;
+    if (res->dU) {  // device result: one copy each, then the instances redone on the host
+        const auto t0 = std::chrono::steady_clock::now();
+        if (res->usz) PQ_HIP(hipMemcpy(U, res->dU, (size_t)res->usz * sizeof(double), hipMemcpyDeviceToHost));
+        if (res->vsz) PQ_HIP(hipMemcpy(V, res->dV, (size_t)res->vsz * sizeof(double), hipMemcpyDeviceToHost));
+        for (size_t x = 0; x < res->inst.size(); ++x) {
+            const Inst& s = res->inst[x];
+            const size_t us = (size_t)s.k * s.m, vs = (size_t)s.k * s.n;
+            if (res->host_inst[x]) {
+                if (us) memcpy(U + uo, s.U.data(), us * sizeof(double));
+                if (vs) memcpy(V + vo, s.V.data(), vs * sizeof(double));
+            }
+            uo += us;
+            vo += vs;
+        }
+        res->dev[3] += std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+        return 0;
+    }
     for (auto& s : res->inst) {
         const size_t us = (size_t)s.k * s.m, vs = (size_t)s.k * s.n;
         if (us) memcpy(U + uo, s.U.data(), us * sizeof(double));
