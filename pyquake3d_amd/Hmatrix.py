@@ -580,8 +580,10 @@ class BlockTree:
             part = dense[s:s + chunk]
             subs = self.src.segments([(b.row_cluster, b.col_cluster) for b in part])
             for b, g in zip(part, subs):
+                # (m, n) views in column-major order, as evaluated and as the
+                # device packing reads them: no host copy
                 for c, name in enumerate(MATRIX_NAMES):
-                    setattr(b, "Mf_" + name, np.ascontiguousarray(g[c]))
+                    setattr(b, "Mf_" + name, g[c])
                 b.judaca = False
                 b.judproc = True
                 self.size += g.nbytes / 2 ** 20

@@ -45,9 +45,21 @@ __global__ void gfsrc_setup_kernel(const double* __restrict__ P, long long n, Sr
     out[j] = g;
 }
 
+// one slip direction per table: strike for sources [0, n), dip for [n, 2n)
+__global__ void gfsrc_setup1_kernel(const double* __restrict__ P, long long n, SrcGeom<1>* __restrict__ out) {
+    long long j = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= 2 * n) return;
+    const long long t = j < n ? j : j - n;
+    const double b[2][1][3] = {{{0.0, 1.0, 0.0}}, {{0.0, 0.0, 1.0}}};
+    SrcGeom<1> g;
+    src_setup<1>(P + 3 * t, P + 3 * (n + t), P + 3 * (2 * n + t), b[j < n ? 0 : 1], g);
+    out[j] = g;
+}
+
 struct GFSource {
     long long nsrc = 0, nrcv = 0;
     SrcGeom<2>* geo = nullptr;
+    SrcGeom<1>* geo1 = nullptr;  // strike-only then dip-only tables (2 nsrc)
     double* xg = nullptr;
     double* ev = nullptr;
     double mu = 0, lam = 0, nu = 0;
@@ -812,7 +824,7 @@ __device__ __forceinline__ int wmin_int(int v) {
 
 // argmax over unused entries with the host loop's semantics: the first
 // maximum of |x| in index order; a NaN after the first valid entry wins.
-__device__ int dargmax_unused(const double* x, const unsigned char* used, int len, double* best) {
+__device__ __forceinline__ int dargmax_unused(const double* x, const unsigned char* used, int len, double* best) {
     const int lane = threadIdx.x & 63;
     double bv = -1.0;
     int bi = INT_MAX, fv = INT_MAX;
@@ -861,7 +873,7 @@ __device__ int dargmax_unused(const double* x, const unsigned char* used, int le
     return bi;
 }
 
-__device__ int dpick_unused(DAca& s, const unsigned char* used, int len) {
+__device__ __forceinline__ int dpick_unused(DAca& s, const unsigned char* used, int len) {
     const int lane = threadIdx.x & 63;
     const int start = (int)(dsplitmix(s.rng) % (unsigned long long)len);
     for (int base = 0; base < len; base += 64) {
@@ -898,30 +910,55 @@ __device__ __forceinline__ void dset_flag(unsigned char* f, int i) {
     __threadfence_block();
 }
 
-__device__ void dresidual_row(const DAca& s, const DView& w, const double* raw, int i, double* out) {
-    for (int j = threadIdx.x & 63; j < s.n; j += 64) {
-        double v = raw[j];
-        for (int l = 0; l < s.k; ++l) {
-            const double a = w.U[(long long)l * s.m + i];
-            if (a == 0.0) continue;
-            v -= a * w.V[(long long)l * s.n + j];
+// lane l's value in every lane (l uniform)
+__device__ __forceinline__ double rd_lane(double v, int l) {
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_readlane((int)(b & 0xffffffffll), l);
+    const int hi = __builtin_amdgcn_readlane((int)(b >> 32), l);
+    return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+
+// out[t] = raw[t] - sum over l < k with c_l != 0 of c_l B[l ldb + t], t < len,
+// the terms in order of l; c_l = C[l ldc] (k <= 64, one per lane).  Four
+// entries per lane per pass keep four loads in flight.
+__device__ __forceinline__ void dresidual(int k, const double* C, long long ldc, const double* B, long long ldb,
+                          const double* raw, int len, double* out) {
+    const int lane = threadIdx.x & 63;
+    const double coef = lane < k ? C[(long long)lane * ldc] : 0.0;
+    for (int base = 0; base < len; base += 256) {
+        double v[4];
+#pragma unroll
+        for (int c = 0; c < 4; ++c) {
+            const int t = base + lane + 64 * c;
+            v[c] = t < len ? raw[t] : 0.0;
         }
-        out[j] = v;
+        for (int l = 0; l < k; ++l) {
+            const double a = rd_lane(coef, l);
+            if (a == 0.0) continue;
+            const double* b = B + (long long)l * ldb;
+#pragma unroll
+            for (int c = 0; c < 4; ++c) {
+                const int t = base + lane + 64 * c;
+                if (t < len) v[c] -= a * b[t];
+            }
+        }
+#pragma unroll
+        for (int c = 0; c < 4; ++c) {
+            const int t = base + lane + 64 * c;
+            if (t < len) out[t] = v[c];
+        }
     }
     __threadfence_block();
 }
 
-__device__ void dresidual_col(const DAca& s, const DView& w, const double* raw, int j, double* out) {
-    for (int i = threadIdx.x & 63; i < s.m; i += 64) {
-        double v = raw[i];
-        for (int l = 0; l < s.k; ++l) {
-            const double a = w.V[(long long)l * s.n + j];
-            if (a == 0.0) continue;
-            v -= a * w.U[(long long)l * s.m + i];
-        }
-        out[i] = v;
-    }
-    __threadfence_block();
+// residual of row i: raw - sum_l U[l m + i] V[l, :]
+__device__ __forceinline__ void dresidual_row(const DAca& s, const DView& w, const double* raw, int i, double* out) {
+    dresidual(s.k, w.U + i, s.m, w.V, s.n, raw, s.n, out);
+}
+
+// residual of column j: raw - sum_l V[l n + j] U[l, :]
+__device__ __forceinline__ void dresidual_col(const DAca& s, const DView& w, const double* raw, int j, double* out) {
+    dresidual(s.k, w.V + j, s.n, w.U, s.m, raw, s.m, out);
 }
 
 __device__ __forceinline__ void dpush(DAca& s, int type, int idx) {
@@ -935,7 +972,7 @@ __device__ __forceinline__ void dpush(DAca& s, int type, int idx) {
     ++s.nq;
 }
 
-__device__ void dstart_refs(DAca& s, const DView& w) {
+__device__ __forceinline__ void dstart_refs(DAca& s, const DView& w) {
     if (s.need_rr) {
         s.iref = dpick_unused(s, w.ur, s.m);
         if (s.iref < 0) {
@@ -959,7 +996,7 @@ __device__ void dstart_refs(DAca& s, const DView& w) {
     s.state = s.nq ? ST_REFS : ST_LEAD;
 }
 
-__device__ void ddecide(DAca& s, const DView& w, double eps) {
+__device__ __forceinline__ void ddecide(DAca& s, const DView& w, double eps) {
     for (int guard = 0; guard < 16; ++guard) {
         const double tiny = 1e-12 * s.scale;
         const double rms = sqrt(s.norm2 / ((double)s.m * (double)s.n));
@@ -1012,7 +1049,7 @@ __device__ void ddecide(DAca& s, const DView& w, double eps) {
     s.state = ST_DONE;
 }
 
-__device__ void dadd_term(DAca& s, const DView& w, double eps) {
+__device__ __forceinline__ void dadd_term(DAca& s, const DView& w, double eps) {
     const int lane = threadIdx.x & 63;
     const double delta = w.r[s.jstar];
     const double tiny = 1e-12 * s.scale;
@@ -1040,8 +1077,25 @@ __device__ void dadd_term(DAca& s, const DView& w, double eps) {
     __threadfence_block();
     uu = wsum(uu);
     vv = wsum(vv);
+    // cross = sum_l (U_l . u)(V_l . v), four terms' dot products per pass
     double cross = 0;
-    for (int l = 0; l < k; ++l) {
+    int l = 0;
+    for (; l + 4 <= k; l += 4) {
+        double a[4] = {0, 0, 0, 0}, b[4] = {0, 0, 0, 0};
+        for (int i = lane; i < m; i += 64) {
+            const double x = u[i];
+#pragma unroll
+            for (int c = 0; c < 4; ++c) a[c] += w.U[(long long)(l + c) * m + i] * x;
+        }
+        for (int j = lane; j < n; j += 64) {
+            const double x = v[j];
+#pragma unroll
+            for (int c = 0; c < 4; ++c) b[c] += w.V[(long long)(l + c) * n + j] * x;
+        }
+#pragma unroll
+        for (int c = 0; c < 4; ++c) cross += wsum(a[c]) * wsum(b[c]);
+    }
+    for (; l < k; ++l) {
         const double* ul = w.U + (long long)l * m;
         const double* vl = w.V + (long long)l * n;
         double a = 0, b = 0;
@@ -1176,9 +1230,14 @@ __global__ __launch_bounds__(1024) void aca_dev_scan_kernel(const long long* __r
 }
 
 // one pair per lane: the requested values of matrix c go to the instance's
-// raw row/column buffer; an instance's first evaluation also takes the largest
-// finite |entry| over all six matrices (the block's scale)
+// raw row/column buffer.  NB = 2 (an instance's first evaluation): both slip
+// directions, and the largest finite |entry| over all six matrices (the
+// block's scale); NB = 1: only the instance's own slip direction, from the
+// strike-only or dip-only table geo1.  The pair's instance is found with two
+// searches per workgroup and one over the few instances the workgroup spans.
+template <int NB>
 __global__ __launch_bounds__(256) void aca_dev_eval_kernel(const SrcGeom<2>* __restrict__ geo,
+                                                           const SrcGeom<1>* __restrict__ geo1, long long nsrc,
                                                            const double* __restrict__ xg,
                                                            const double* __restrict__ ev,
                                                            const int* __restrict__ ridx, const int* __restrict__ sidx,
@@ -1187,17 +1246,45 @@ __global__ __launch_bounds__(256) void aca_dev_eval_kernel(const SrcGeom<2>* __r
                                                            DAca* __restrict__ inst, const long long* __restrict__ off,
                                                            long long ni, double mu, double lam, double nu,
                                                            int halfspace, double* __restrict__ dbuf) {
+    __shared__ long long s_lo, s_cnt;
+    __shared__ long long s_off[257];
     const long long total = off[ni];
-    const long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (q >= total) return;
-    long long lo = 0, hi = ni - 1;  // last instance with off <= q (it has pairs)
-    while (lo < hi) {
-        const long long mid = (lo + hi + 1) >> 1;
-        if (off[mid] <= q) lo = mid;
-        else hi = mid - 1;
+    const long long q0 = (long long)blockIdx.x * 256, q = q0 + threadIdx.x;  // q0 < total (grid fits)
+    auto last_le = [&](long long qq, long long lo, long long hi) {  // last x in [lo, hi] with off[x] <= qq
+        while (lo < hi) {
+            const long long mid = (lo + hi + 1) >> 1;
+            if (off[mid] <= qq) lo = mid;
+            else hi = mid - 1;
+        }
+        return lo;
+    };
+    if (threadIdx.x == 0) {
+        const long long a = last_le(q0, 0, ni - 1);
+        const long long b = last_le(q0 + 255 < total ? q0 + 255 : total - 1, a, ni - 1);
+        s_lo = a;
+        s_cnt = b - a + 1;
     }
-    const DAca& s = inst[lo];
-    long long p = q - off[lo];
+    __syncthreads();
+    const long long lo0 = s_lo, cnt = s_cnt;
+    if (cnt <= 256) {
+        if (threadIdx.x < cnt) s_off[threadIdx.x] = off[lo0 + threadIdx.x];
+        __syncthreads();
+    }
+    if (q >= total) return;
+    long long x;
+    if (cnt <= 256) {
+        int lo = 0, hi = (int)cnt - 1;
+        while (lo < hi) {
+            const int mid = (lo + hi + 1) >> 1;
+            if (s_off[mid] <= q) lo = mid;
+            else hi = mid - 1;
+        }
+        x = lo0 + lo;
+    } else {
+        x = last_le(q, lo0, lo0 + cnt - 1);
+    }
+    const DAca& s = inst[x];
+    long long p = q - off[x];
     const long long len0 = s.qt[0] == 0 ? s.n : s.m;
     const int t = p < len0 ? 0 : 1;
     if (t) p -= len0;
@@ -1208,32 +1295,43 @@ __global__ __launch_bounds__(256) void aca_dev_eval_kernel(const SrcGeom<2>* __r
     double e[9];
 #pragma unroll
     for (int k = 0; k < 9; ++k) e[k] = ev[9 * (long long)r + k];
-    double G[2][9];
-    td_pair_grad<double, 2, false>(geo[src], X, Y, Z, nu, halfspace != 0, G);
-    double vals[6];
+    double v;
+    if constexpr (NB == 2) {
+        double G[2][9];
+        td_pair_grad<double, 2, false>(geo[src], X, Y, Z, nu, halfspace != 0, G);
+        double vals[6];
 #pragma unroll
-    for (int b = 0; b < 2; ++b) {
+        for (int bb = 0; bb < 2; ++bb) {
+            double eps[6], sig[6], tr[3];
+            grad_to_strain_stress<double>(G[bb], mu, lam, eps, sig);
+            stress_to_traction<double>(sig, e, tr);
+#pragma unroll
+            for (int k = 0; k < 3; ++k) vals[bb * 3 + k] = tr[k];
+        }
+        v = vals[0];
+#pragma unroll
+        for (int k = 1; k < 6; ++k)
+            if (k == s.c) v = vals[k];
+        if (s.first) {
+            double mx = 0;
+#pragma unroll
+            for (int k = 0; k < 6; ++k) {
+                const double a = fabs(vals[k]);
+                if (a == a && a > mx) mx = a;
+            }
+            atomicMax(&inst[x].scale_bits, (unsigned long long)__double_as_longlong(mx));
+        }
+    } else {
+        double G[1][9];
+        td_pair_grad<double, 1, false>(geo1[(s.c >= 3 ? nsrc : 0) + src], X, Y, Z, nu, halfspace != 0, G);
         double eps[6], sig[6], tr[3];
-        grad_to_strain_stress<double>(G[b], mu, lam, eps, sig);
+        grad_to_strain_stress<double>(G[0], mu, lam, eps, sig);
         stress_to_traction<double>(sig, e, tr);
-#pragma unroll
-        for (int k = 0; k < 3; ++k) vals[b * 3 + k] = tr[k];
+        const int cc = s.c >= 3 ? s.c - 3 : s.c;
+        v = cc == 0 ? tr[0] : cc == 1 ? tr[1] : tr[2];
     }
-    double v = vals[0];
-#pragma unroll
-    for (int k = 1; k < 6; ++k)
-        if (k == s.c) v = vals[k];
     if (type == 0) dbuf[s.offN + 2 * (long long)s.n + p] = v;   // rawR
     else dbuf[s.offM + 2 * (long long)s.m + p] = v;             // rawC
-    if (s.first) {
-        double mx = 0;
-#pragma unroll
-        for (int k = 0; k < 6; ++k) {
-            const double a = fabs(vals[k]);
-            if (a == a && a > mx) mx = a;
-        }
-        atomicMax(&inst[lo].scale_bits, (unsigned long long)__double_as_longlong(mx));
-    }
 }
 
 // ---- recompression on the device: the algorithm of recompress_factors
@@ -1643,6 +1741,7 @@ int pq3d_gfsrc_create(void** out, const double* P1, const double* P2, const doub
     g->halfspace = halfspace;
     double* dP = nullptr;
     hipError_t e = hipMalloc((void**)&g->geo, (size_t)nsrc * sizeof(SrcGeom<2>));
+    if (e == hipSuccess) e = hipMalloc((void**)&g->geo1, (size_t)nsrc * 2 * sizeof(SrcGeom<1>));
     if (e == hipSuccess) e = hipMalloc((void**)&g->xg, (size_t)nrcv * 12 * sizeof(double));
     if (e == hipSuccess) e = hipMalloc((void**)&dP, (size_t)nsrc * 9 * sizeof(double));
     if (e == hipSuccess) {
@@ -1658,11 +1757,17 @@ int pq3d_gfsrc_create(void** out, const double* P1, const double* P2, const doub
                            g->geo);
         e = hipGetLastError();
     }
+    if (e == hipSuccess) {
+        hipLaunchKernelGGL(gfsrc_setup1_kernel, dim3((unsigned)((2 * nsrc + 255) / 256)), dim3(256), 0, 0, dP, nsrc,
+                           g->geo1);
+        e = hipGetLastError();
+    }
     if (e == hipSuccess) e = hipDeviceSynchronize();
     if (dP) (void)hipFree(dP);
     if (e != hipSuccess) {
         set_error("pq3d_gfsrc_create: %s", hipGetErrorString(e));
         (void)hipFree(g->geo);
+        (void)hipFree(g->geo1);
         (void)hipFree(g->xg);
         delete g;
         return -1;
@@ -1676,6 +1781,7 @@ int pq3d_gfsrc_destroy(void* h) {
     GFSource* g = (GFSource*)h;
     if (!g) return 0;
     (void)hipFree(g->geo);
+    (void)hipFree(g->geo1);
     (void)hipFree(g->xg);
     (void)hipFree(g->seg);
     (void)hipFree(g->out);
@@ -1931,9 +2037,14 @@ int pq3d_aca_run_device(void* h, long long nb, const long long* row_off, const i
             break;
         }
         ACA_TRY(hipEventRecord(ea, 0));
-        hipLaunchKernelGGL(aca_dev_eval_kernel, dim3((unsigned)nblk), dim3(256), 0, 0, g->geo, g->xg, g->ev, d_idx,
-                           d_idx + nr, d_offs, d_offs + nb + 1, d_inst, d_off, ni, g->mu, g->lam, g->nu,
-                           g->halfspace, d_buf);
+        if (cnt.sweeps == 1)  // every instance's first evaluation: both slip directions, the scale
+            hipLaunchKernelGGL(aca_dev_eval_kernel<2>, dim3((unsigned)nblk), dim3(256), 0, 0, g->geo, g->geo1,
+                               g->nsrc, g->xg, g->ev, d_idx, d_idx + nr, d_offs, d_offs + nb + 1, d_inst, d_off, ni,
+                               g->mu, g->lam, g->nu, g->halfspace, d_buf);
+        else
+            hipLaunchKernelGGL(aca_dev_eval_kernel<1>, dim3((unsigned)nblk), dim3(256), 0, 0, g->geo, g->geo1,
+                               g->nsrc, g->xg, g->ev, d_idx, d_idx + nr, d_offs, d_offs + nb + 1, d_inst, d_off, ni,
+                               g->mu, g->lam, g->nu, g->halfspace, d_buf);
         ACA_TRY(hipGetLastError());
         ACA_TRY(hipEventRecord(eb, 0));
         ACA_TRY(hipEventSynchronize(eb));
