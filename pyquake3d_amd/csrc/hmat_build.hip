@@ -1928,6 +1928,9 @@ int pq3d_aca_run_device(void* h, long long nb, const long long* row_off, const i
     if (int rc = aca_check(g, nb, row_off, row_idx, col_off, col_idx, eps)) return rc;
     if (kcap <= 0) kcap = 48;
     PQ_CHECK(kcap <= 64, "pq3d_aca_run_device: kcap %d above 64 (one SVD column per lane)", kcap);
+    if (nb == 0)  // nothing to launch: the host loop's empty result
+        return pq3d_aca_run(h, nb, row_off, row_idx, col_off, col_idx, eps, recompress_eps, max_rank, seed, ranks,
+                            result);
     using clk = std::chrono::steady_clock;
     const auto t_start = clk::now();
     AcaResult* res = new AcaResult();

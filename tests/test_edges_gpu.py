@@ -7,7 +7,9 @@
   ensemble GEMM against the FP64 product of the FP32 matrix (2e-6 of the
   magnitude sum) plus every member's steps against the oracle (1e-6);
 * a mesh size that is not a multiple of any tile (n = 70): dense matvec per
-  row within 1e-12 of sum |a_ij u_j| (FP64 storage).
+  row within 1e-12 of sum |a_ij u_j| (FP64 storage);
+* ACA with no blocks, and with blocks smaller than a wave, on the device
+  pivoting path.
 """
 import ctypes
 
@@ -132,3 +134,33 @@ def test_ragged_size_dense_product():
         got = s.get(name)
         assert (np.abs(got - want) / np.where(mag > 0, mag, 1.0)).max() < 1e-12, name
     s.close()
+
+
+def test_aca_with_no_blocks_and_blocks_smaller_than_a_wave():
+    """No blocks: an empty result on both paths.  Blocks of 3 x 5 and 2 x 2
+    elements (fewer rows than a wave's lanes) and a well-separated 40 x 40:
+    the device pivoting meets the ACA bar of the H-matrix tests, with ranks
+    equal to the host loop's."""
+    from pyquake3d_amd import Hmatrix as H, readmsh, synth
+    nodes_km, tri = synth.planar_fault_mesh(20, 10, 40.0, 20.0)
+    nodes = nodes_km * 1e3
+    ev, xg = readmsh.get_eleVec(nodes, tri, False)
+    src = H.GFSource(nodes, tri, ev, xg, MU, LAM, True)
+    for device in (False, True):
+        out, st = H.aca_blocks(src, [], eps=1e-4, device=device)
+        assert out == [] and st["unconverged"] == 0
+    n = len(tri)
+    blocks = [(np.arange(3), np.arange(n - 5, n)), (np.array([7, 9]), np.array([n - 1, n - 3])),
+              (np.arange(40), np.arange(n - 40, n))]
+    dense = src.segments(blocks)
+    host, _ = H.aca_blocks(src, blocks, eps=1e-4)
+    dev, st = H.aca_blocks(src, blocks, eps=1e-4, device=True)
+    assert st["pivoting"] == "device"
+    for A, ph, pd in zip(dense, host, dev):
+        strongest = max(np.linalg.norm(A[c]) for c in range(6))
+        for c in range(6):
+            U, V = pd[c]
+            err = np.linalg.norm(A[c] - U @ V)
+            assert err <= 10 * 1e-4 * np.linalg.norm(A[c]) + 1e-8 * strongest, (A[c].shape, c, err)
+            assert U.shape[1] == ph[c][0].shape[1], (A[c].shape, c)
+    src.close()
