@@ -11,14 +11,20 @@ Legs
   (a-b ~ U[-0.006,-0.002] on the patch, Dc = 0.14 exp(0.3 N), sigma_n ~
   U[20,30] MPa) share the FP32 matrices; the traction rates of all members
   come from one tall-skinny GEMM on MFMA per global step, and every member
-  takes one accepted adaptive RK45 step with its own step size.
+  takes one accepted adaptive RK45 step with its own step size.  The GEMM
+  splits every FP32 operand exactly into three BF16 parts and accumulates the
+  six leading products in FP32 on the BF16 MFMA units (the same ensemble with
+  the FP32-MFMA GEMM is reported beside it).
   value = accepted member-steps per second (64 x global steps / time).
 * plain: one simulation (aging law, Dc lognormal sigma 0.2, V = Vpl = 1e-9
   m/s, steady state) with FP32 and with FP64 matrix storage; member-steps/s.
-* roofline: the ensemble GEMM (dominant kernel of `value`), 2*6*N^2*64 flops
-  per launch over its mean launch time from HIP events, against the 157.3
-  TFLOP/s FP32 MFMA peak; the plain FP32 matvec against 8 TB/s HBM is reported
-  beside it.
+* roofline: the ensemble GEMM (dominant kernel of `value`): with the BF16
+  split it executes 6 x 2 x 64 = 768 flops per 4-byte matrix entry, 192
+  flop/B, below the BF16 MFMA ridge (2.5 PFLOP/s / 8 TB/s = 312 flop/B), so it
+  is HBM-bound: algorithmic bytes (matrices, slip rates, outputs) per launch over
+  the mean launch time from HIP events, against 8 TB/s; the FP32-MFMA GEMM's
+  roofline (2 nmat N^2 64 flops against 157.3 TFLOP/s) and the plain FP32
+  matvec against 8 TB/s are reported beside it.
 * cpu_baseline: the NumPy FP64 restatement of the reference's dense step
   (oracle/qdsim_oracle.py: six np.dot per step, BLAS on all host cores) on the
   same 20k FP64 matrices, over a bounded number of steps.
@@ -584,16 +590,20 @@ def main():
     init_members(ens, mcols, V0, VPL, MU, CS)
     kc0 = ens.kc
     tuning["ens_gemm"] = _tuned(ens, autotune.tune_ensemble(ens, use_cache=False), kc0)
+    precision, split = ens.precision
     ens_ms, accepted = time_ensemble(ens, args.steps, args.warmup, args.graph_steps)
     value = accepted / (ens_ms * 1e-3)
     gemm_ms = ens.gemm(reps=10)
     mem = ens.members()
     gemm_flops = 2.0 * nmat * N * N * MEMBERS
     gemm_tf = gemm_flops / (gemm_ms * 1e-3) / 1e12
-    log(f"[bench] ensemble: {value:.1f} member-steps/s ({args.steps} global steps in {ens_ms:.1f} ms); "
-        f"GEMM {gemm_ms:.3f} ms = {gemm_tf:.1f} TFLOP/s")
+    gemm_bytes = nmat * N * N * 4.0 + 2.0 * N * MEMBERS * 4 + 3.0 * N * MEMBERS * 4
+    gemm_gbs = gemm_bytes / (gemm_ms * 1e-3) / 1e9
+    log(f"[bench] ensemble ({precision}): {value:.1f} member-steps/s ({args.steps} global steps in {ens_ms:.1f} ms); "
+        f"GEMM {gemm_ms:.3f} ms = {gemm_tf:.1f} TFLOP/s of the product, {gemm_gbs:.0f} GB/s")
     ens_info = {"members": MEMBERS, "global_steps": args.steps, "ms_per_global_step": ens_ms / args.steps,
-                "gemm_ms": gemm_ms, "member_sim_time_s_min": float(mem["time"].min()),
+                "gemm_ms": gemm_ms, "gemm_precision": precision, "gemm_bf16_split": split,
+                "member_sim_time_s_min": float(mem["time"].min()),
                 "member_sim_time_s_max": float(mem["time"].max()), "failed_members": int((mem["status"] != 0).sum())}
     # traction rates of the members' current slip rates through the batched
     # path (FP32 storage, MFMA GEMM), checked below against the FP64 product
@@ -601,6 +611,25 @@ def main():
     gemm_out = ens.gemm_result()
     member_u = {k: ens.get(k) for k in ("slipv1", "slipv2", "rake0")}
     ens.close()
+    # the same ensemble with the FP32-MFMA GEMM
+    e32 = Ensemble(N, MEMBERS, matrices_from=sim, precision="fp32")
+    init_members(e32, mcols, V0, VPL, MU, CS)
+    tuning["ens_gemm_fp32"] = _tuned(e32, autotune.tune_ensemble(e32, use_cache=False), kc0)
+    f32_ms, f32_acc = time_ensemble(e32, args.steps, args.warmup, args.graph_steps)
+    f32_gemm_ms = e32.gemm(reps=10)
+    f32_tf = gemm_flops / (f32_gemm_ms * 1e-3) / 1e12
+    e32.gemm()
+    gemm_out32 = e32.gemm_result()
+    member_u32 = {k: e32.get(k) for k in ("slipv1", "slipv2", "rake0")}
+    e32.close()
+    fp32_gemm = {"value": f32_acc / (f32_ms * 1e-3), "gemm_ms": f32_gemm_ms,
+                 "roofline": {"bound": "mfma", "achieved": f32_tf, "peak": FP32_MFMA_PEAK_TFLOPS, "unit": "TFLOP/s",
+                              "frac": f32_tf / FP32_MFMA_PEAK_TFLOPS,
+                              "kernel": "ens_gemm_kernel (v_mfma_f32_16x16x4f32)",
+                              "pmc": _pmc_mfma("ens_gemm_20k_mfma_pmc.json"),
+                              "traffic": _pmc_traffic("ens_gemm_20k_traffic.json")}}
+    log(f"[bench] same ensemble, FP32-MFMA GEMM: {fp32_gemm['value']:.1f} member-steps/s, GEMM {f32_gemm_ms:.3f} ms "
+        f"= {f32_tf:.1f} TFLOP/s")
     elision = {"zero_matrices": [n for k, n in enumerate(MATRIX_NAMES) if zmask >> k & 1],
                "matrices_streamed": nmat}
     if skip:
@@ -631,7 +660,9 @@ def main():
     mv64_ms = s64.time_matvec(10)
     log(f"[bench] plain FP64: {plain64:.1f} member-steps/s; matvec {mv64_ms:.3f} ms")
     terr = traction_error(s64, gemm_out, member_u)
-    del gemm_out, member_u
+    terr32 = traction_error(s64, gemm_out32, member_u32)
+    fp32_gemm["traction_error"] = {k: terr32[k] for k in ("max_rel_l2", "max_rel_max")}
+    del gemm_out, member_u, gemm_out32, member_u32
     log(f"[bench] batched traction rates vs FP64 product: relative L2 error {terr['max_rel_l2']:.2e}, "
         f"max-norm {terr['max_rel_max']:.2e} over {len(terr['members'])} members")
     cpu = None
@@ -677,16 +708,26 @@ def main():
         "data": "synthetic",
         "config": {"workload": "batched ensemble: 64 members (seeded a-b, Dc, sigma_n) on a planar vertical fault, "
                                "40x40 km, 100x100 grid -> 20,000 TDs, half space; FP64 GF assembly, FP32 GF storage, "
-                               "GEMM on FP32 MFMA; aging-law RSF with radiation damping, adaptive RK45 per member; "
-                               "a step = one global step (one accepted step of every member)",
+                               "GEMM with FP32 operands split exactly into three BF16 parts (six products, FP32 "
+                               "accumulation, BF16 MFMA); aging-law RSF with radiation damping, adaptive RK45 per "
+                               "member; a step = one global step (one accepted step of every member)",
                    "n_elements": N, "members": MEMBERS, "gf_storage": "fp32", "state_dtype": "f64",
                    "parallelism": "single GPU"},
-        "roofline": {"bound": "mfma", "achieved": gemm_tf, "peak": FP32_MFMA_PEAK_TFLOPS, "unit": "TFLOP/s",
-                     "frac": gemm_tf / FP32_MFMA_PEAK_TFLOPS, "traffic": _pmc_traffic("ens_gemm_20k_traffic.json"),
-                     "kernel": f"ens_gemm_kernel (v_mfma_f32_16x16x4f32, {nmat} x 20k x 20k FP32 by 20k x 64; "
-                               f"{6 - nmat} all-zero matrices skipped)",
-                     "flops_per_launch": gemm_flops, "avg_launch_ms": gemm_ms,
-                     "pmc": _pmc_mfma("ens_gemm_20k_mfma_pmc.json")},
+        "roofline": ({"bound": "hbm", "achieved": gemm_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                      "frac": gemm_gbs / HBM_PEAK_GBS, "traffic": _pmc_traffic("ens_gemm3_20k_traffic.json"),
+                      "kernel": f"ens_gemm3_kernel (FP32 operands split into three BF16 parts, six products on "
+                                f"v_mfma_f32_16x16x32_bf16; {nmat} x 20k x 20k FP32 by 20k x 64; "
+                                f"{6 - nmat} all-zero matrices skipped)",
+                      "alg_bytes_per_launch": gemm_bytes, "avg_launch_ms": gemm_ms,
+                      "product_tflops": gemm_tf, "pmc": _pmc_mfma("ens_gemm3_20k_mfma_pmc.json")}
+                     if split else
+                     {"bound": "mfma", "achieved": gemm_tf, "peak": FP32_MFMA_PEAK_TFLOPS, "unit": "TFLOP/s",
+                      "frac": gemm_tf / FP32_MFMA_PEAK_TFLOPS, "traffic": _pmc_traffic("ens_gemm_20k_traffic.json"),
+                      "kernel": f"ens_gemm_kernel (v_mfma_f32_16x16x4f32, {nmat} x 20k x 20k FP32 by 20k x 64; "
+                                f"{6 - nmat} all-zero matrices skipped)",
+                      "flops_per_launch": gemm_flops, "avg_launch_ms": gemm_ms,
+                      "pmc": _pmc_mfma("ens_gemm_20k_mfma_pmc.json")}),
+        "fp32_mfma_gemm": fp32_gemm,
         "cpu_baseline": cpu,
         "plain_fp32_member_steps_per_s": plain32,
         "plain_fp64_member_steps_per_s": plain64,

@@ -186,11 +186,18 @@ int pq3d_ens_reset_member(void* ens, int member, double h0, double t_end, int ac
 int pq3d_ens_gemm(void* ens, int reps, float* ms_per_launch);
 int pq3d_ens_gemm_result(void* ens, double* host_3Bn);
 /* split-K tile width of the GEMM (64 receivers x 64 members x kc sources per
- * wave; kc rounded up to a multiple of 4); time_product = operand prep + GEMM
- * + one pass over the partial sums */
+ * wave; kc rounded up to a multiple of 4, of 32 in precision mode 1);
+ * time_product = operand prep + GEMM + one pass over the partial sums */
 int pq3d_ens_set_kc(void* ens, int kc);
 int pq3d_ens_get_kc(void* ens, int* kc, int* nchunk);
 int pq3d_ens_time_product(void* ens, int reps, float* ms_per_product);
+/* GEMM arithmetic (no reference counterpart): mode 0 FP32 MFMA; mode 1 each
+ * FP32 operand split exactly into three BF16 parts and the six leading
+ * products accumulated in FP32 on the BF16 MFMA units (dropped terms 2^-24
+ * of each product and below).  Mode 1 applies to the two-component GEMM
+ * (Bs = Bd = 0); split_active says whether it is the one launched */
+int pq3d_ens_set_precision(void* ens, int mode);
+int pq3d_ens_get_precision(void* ens, int* mode, int* split_active);
 /* zero-matrix structure of the GEMM's six matrices, as for the dense product */
 int pq3d_ens_scan_structure(void* ens, int* mask);
 int pq3d_ens_set_structure(void* ens, int mask);

@@ -81,8 +81,14 @@ def _nq(obj):
     return 2 if (st is not None and st[1]) else 3
 
 
+def _split(ens):
+    """The ensemble runs the BF16-split GEMM (32-source steps)."""
+    p = getattr(ens, "precision", None)
+    return bool(p[1]) if p is not None else False
+
+
 def ensemble_key(ens):
-    return f"{_device_key()}|ens_gemm|n{ens.n}|B{ens.B}|nq{_nq(ens)}"
+    return f"{_device_key()}|ens_gemm|n{ens.n}|B{ens.B}|nq{_nq(ens)}" + ("|bf16x3" if _split(ens) else "")
 
 
 def dense_key(sim):
@@ -90,13 +96,14 @@ def dense_key(sim):
             f"|nq{_nq(sim)}")
 
 
-def ensemble_candidates(n):
-    """kc for 1..32 column chunks, rounded up to the 4-deep MFMA step (32
-    chunks bound the partial-sum memory an ensemble is budgeted for)."""
+def ensemble_candidates(n, step=4):
+    """kc for 1..32 column chunks, rounded up to the MFMA step (4 sources,
+    32 for the BF16-split GEMM; 32 chunks bound the partial-sum memory an
+    ensemble is budgeted for)."""
     out = []
     for nc in range(1, 33):
-        kc = (math.ceil(n / nc) + 3) // 4 * 4
-        if kc >= 4 and kc not in out:
+        kc = (math.ceil(n / nc) + step - 1) // step * step
+        if kc >= step and kc not in out:
             out.append(kc)
     return out
 
@@ -130,7 +137,8 @@ def tune_ensemble(ens, candidates=None, reps=4, trials=2, use_cache=True):
     """Measure split-K widths for an ``Ensemble`` (its matrices resident),
     apply the fastest and cache it.  Returns {"kc", "ms", "table", "key"}."""
     key = ensemble_key(ens)
-    cands = list(candidates) if candidates is not None else ensemble_candidates(ens.n)
+    cands = (list(candidates) if candidates is not None
+             else ensemble_candidates(ens.n, 32 if _split(ens) else 4))
     (kc, ms), table = _sweep(ens, cands, reps, trials)
     entry = {"kc": kc, "ms": ms, "candidates": len(table)}
     if use_cache:

@@ -40,6 +40,25 @@
 namespace pq3d {
 
 typedef float v4f __attribute__((ext_vector_type(4)));
+typedef __bf16 bf8 __attribute__((ext_vector_type(8)));
+
+// x = hi + mid + lo with BF16 parts, exactly: hi is x truncated to its top
+// 8 significant bits (the high half of its FP32 pattern), mid the next 8 of
+// r = x - hi, lo the last 8 (r - mid).  The residuals are exact in FP32, and
+// each part is the high half-word of x, r or r - mid.
+struct Split3f {
+    float x, r, s;  // the parts are the high halves of these three patterns
+};
+__device__ __forceinline__ Split3f split3f(float x) {
+    const float h = __uint_as_float(__float_as_uint(x) & 0xffff0000u);
+    const float r = x - h;
+    const float m = __uint_as_float(__float_as_uint(r) & 0xffff0000u);
+    return Split3f{x, r, r - m};
+}
+// BF16 pair (low element a, high element b) from the high halves of two floats
+__device__ __forceinline__ unsigned pack_hi(float a, float b) {
+    return __builtin_amdgcn_perm(__float_as_uint(b), __float_as_uint(a), 0x07060302u);
+}
 
 enum EField : int {
     E_TNO = 0, E_TT1O, E_TT2O, E_STATE, E_SLIPV1, E_SLIPV2, E_SLIPV, E_SLIP1, E_SLIP2, E_SLIP, E_TT, E_RAKE,
@@ -92,7 +111,8 @@ __global__ __launch_bounds__(256) void ens_prep_u_kernel(const double* __restric
                                                          const double* __restrict__ s0, double Vpl, long long n,
                                                          int B, MemberCtrl* __restrict__ mc,
                                                          EnsCtrl* __restrict__ ctrl, float* __restrict__ U1,
-                                                         float* __restrict__ U2) {
+                                                         float* __restrict__ U2, uint4* __restrict__ planes,
+                                                         long long plane_len) {
     __shared__ float t1[64][65], t2[64][65];  // [member][source], padded against bank conflicts
     const int tid = threadIdx.x;
     if (blockIdx.x == 0) {
@@ -119,6 +139,28 @@ __global__ __launch_bounds__(256) void ens_prep_u_kernel(const double* __restric
         if (j < n) {
             U1[(size_t)j * 64 + m] = t1[m][jj];
             U2[(size_t)j * 64 + m] = t2[m][jj];
+        }
+    }
+    if (planes) {
+        // three-part BF16 split, MFMA operand order: plane (array, part) holds,
+        // per 8 sources and member, the 8 sources' parts (16 bytes); sources
+        // past n are zero
+        for (int it = tid; it < 2 * 8 * 64; it += 256) {
+            const int arr = it >> 9, g8 = (it >> 6) & 7, m = it & 63;
+            const float(*t)[65] = arr ? t2 : t1;
+            unsigned h[4], md[4], lo[4];
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                const Split3f a = split3f(t[m][8 * g8 + 2 * k]), b = split3f(t[m][8 * g8 + 2 * k + 1]);
+                h[k] = pack_hi(a.x, b.x);
+                md[k] = pack_hi(a.r, b.r);
+                lo[k] = pack_hi(a.s, b.s);
+            }
+            const size_t at = ((size_t)(j0 / 8 + g8) * 64 + m);
+            uint4* base = planes + (size_t)arr * 3 * plane_len;
+            base[at] = make_uint4(h[0], h[1], h[2], h[3]);
+            base[plane_len + at] = make_uint4(md[0], md[1], md[2], md[3]);
+            base[2 * plane_len + at] = make_uint4(lo[0], lo[1], lo[2], lo[3]);
         }
     }
 }
@@ -231,6 +273,154 @@ __global__ __launch_bounds__(64) void ens_gemm_kernel(const float* __restrict__ 
         ens_gemm_tile<2, NQ>(A1s, A2s, Bs, A1d, A2d, Bd, ld, n, kc, U1, U2, partial, npad);
     else
         ens_gemm_tile<1, NQ>(A1s, A2s, Bs, A1d, A2d, Bd, ld, n, kc, U1, U2, partial, npad);
+}
+
+// ---------------------------------------------------------------------------
+// The same GEMM for the two shear components (Bs = Bd = 0) on the BF16 MFMA
+// units (v_mfma_f32_16x16x32_bf16): every FP32 operand is split into three
+// BF16 parts (hi + mid + lo, exact), and the six products hi.hi, hi.mid,
+// mid.hi, hi.lo, lo.hi, mid.mid are accumulated in FP32; the dropped ones are
+// 2^-24 of the product and below.  u's parts come from ens_prep_u_kernel's
+// planes; A's are split here.  Lane l feeds row 4 (l % 16) + i of subtile i
+// and member 4 (l % 16) + j of subtile j, with sources 8 (l / 16) + t,
+// t < 8, of each 32-source step: eight 16-byte loads per matrix and step, and
+// one 16-byte load per member subtile and part.  One wave per workgroup,
+// occupancy 1: the strike half (A1s, A2s with u1) and the dip half (A1d, A2d
+// with u2) of a step alternate, each half's next loads issued as soon as its
+// registers are free.  kc is a multiple of 32.
+template <int L>
+__device__ __forceinline__ void ens_gemm3_half(const float4 (&a)[2][8], const bf8 (&uf)[4][3], v4f (&acc)[3][4][4]) {
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        // subtile i of both matrices: element t of a fragment = source 8 (lane / 16) + t
+        bf8 f[2][3];
+#pragma unroll
+        for (int q = 0; q < 2; ++q) {
+            unsigned h[4], m[4], l[4];
+#pragma unroll
+            for (int t = 0; t < 4; ++t) {
+                const float4 v0 = a[q][2 * t], v1 = a[q][2 * t + 1];
+                const float x0 = i == 0 ? v0.x : i == 1 ? v0.y : i == 2 ? v0.z : v0.w;
+                const float x1 = i == 0 ? v1.x : i == 1 ? v1.y : i == 2 ? v1.z : v1.w;
+                const Split3f s0 = split3f(x0), s1 = split3f(x1);
+                h[t] = pack_hi(s0.x, s1.x);
+                m[t] = pack_hi(s0.r, s1.r);
+                l[t] = pack_hi(s0.s, s1.s);
+            }
+            const uint4 hv = make_uint4(h[0], h[1], h[2], h[3]), mv = make_uint4(m[0], m[1], m[2], m[3]),
+                        lv = make_uint4(l[0], l[1], l[2], l[3]);
+            f[q][0] = *reinterpret_cast<const bf8*>(&hv);
+            f[q][1] = *reinterpret_cast<const bf8*>(&mv);
+            f[q][2] = *reinterpret_cast<const bf8*>(&lv);
+        }
+        // six products hi.hi, hi.mid, mid.hi, hi.lo, lo.hi, mid.mid; 2 L
+        // independent accumulators between dependent MFMAs
+#pragma unroll
+        for (int p = 0; p < 6; ++p) {
+            const int pa = (p == 0 || p == 1 || p == 3) ? 0 : (p == 2 || p == 5) ? 1 : 2;
+            const int pu = (p == 0 || p == 2 || p == 4) ? 0 : (p == 1 || p == 5) ? 1 : 2;
+#pragma unroll
+            for (int q = 0; q < 2; ++q)
+#pragma unroll
+                for (int j = 0; j < L; ++j)
+                    acc[q][i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(f[q][pa], uf[j][pu], acc[q][i][j], 0, 0, 0);
+        }
+    }
+}
+
+template <int L>
+__device__ __forceinline__ void ens_gemm3_tile(const float* __restrict__ A1s, const float* __restrict__ A2s,
+                                               const float* __restrict__ A1d, const float* __restrict__ A2d,
+                                               long long ld, long long n, int kc, const uint4* __restrict__ planes,
+                                               long long plane_len, float* __restrict__ partial,
+                                               long long npad) {
+    const int lane = threadIdx.x & 63;
+    const long long r0 = (long long)blockIdx.x * 64;
+    const long long k0 = (long long)blockIdx.y * kc;
+    long long k1 = k0 + kc;
+    if (k1 > n) k1 = n;
+    const int lr = lane & 15, g = lane >> 4;
+    const long long row = r0 + 4 * lr < ld ? r0 + 4 * lr : ld - 4;
+    v4f acc[3][4][4];
+#pragma unroll
+    for (int q = 0; q < 3; ++q)
+#pragma unroll
+        for (int i = 0; i < 4; ++i)
+#pragma unroll
+            for (int j = 0; j < 4; ++j) acc[q][i][j] = v4f{0.f, 0.f, 0.f, 0.f};
+    float4 as[2][8], ad[2][8];
+    bf8 us[4][3], ud[4][3];
+    // sources past n read column n - 1 (their u parts are zero)
+    auto load_a = [&](float4 (&a)[2][8], const float* M0, const float* M1, long long kb) {
+#pragma unroll
+        for (int t = 0; t < 8; ++t) {
+            long long k = kb + 8 * g + t;
+            k = k < n ? k : n - 1;
+            a[0][t] = *reinterpret_cast<const float4*>(M0 + (size_t)k * ld + row);
+            a[1][t] = *reinterpret_cast<const float4*>(M1 + (size_t)k * ld + row);
+        }
+    };
+    auto load_u = [&](bf8 (&uf)[4][3], int arr, long long kb) {
+        const uint4* base = planes + (size_t)arr * 3 * plane_len + ((size_t)(kb / 8 + g) * 64 + 4 * lr);
+#pragma unroll
+        for (int j = 0; j < L; ++j)
+#pragma unroll
+            for (int p = 0; p < 3; ++p) {
+                const uint4 v = base[(size_t)p * plane_len + j];
+                uf[j][p] = *reinterpret_cast<const bf8*>(&v);
+            }
+    };
+    load_a(as, A1s, A2s, k0);
+    load_u(us, 0, k0);
+    load_a(ad, A1d, A2d, k0);
+    load_u(ud, 1, k0);
+    for (long long kb = k0; kb < k1; kb += 32) {
+        const bool more = kb + 32 < k1;
+        ens_gemm3_half<L>(as, us, acc);   // A1s u1 -> q 0, A2s u1 -> q 1
+        if (more) {
+            load_a(as, A1s, A2s, kb + 32);
+            load_u(us, 0, kb + 32);
+        }
+        ens_gemm3_half<L>(ad, ud, acc);   // A1d u2, A2d u2 into the same q
+        if (more) {
+            load_a(ad, A1d, A2d, kb + 32);
+            load_u(ud, 1, kb + 32);
+        }
+    }
+    // C fragment as in ens_gemm_tile; the normal component is written as zeros
+#pragma unroll
+    for (int q = 0; q < 3; ++q)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const int member = 4 * lr + j;
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                const long long prow = r0 + 4 * (4 * g + v);
+                float4 o = make_float4(acc[q][0][j][v], acc[q][1][j][v], acc[q][2][j][v], acc[q][3][j][v]);
+                if (q >= 2) o = make_float4(0.f, 0.f, 0.f, 0.f);
+                *reinterpret_cast<float4*>(partial + (((size_t)blockIdx.y * 3 + q) * 64 + member) * npad + prow) = o;
+            }
+        }
+}
+
+// grid: (ceil(n / 64), nchunk); block 64.  Requires B == 64, ld % 4 == 0,
+// kc % 32 == 0 and Bs = Bd = 0.  Idle member subtiles are skipped as in
+// ens_gemm_kernel.
+__global__ __launch_bounds__(64) void ens_gemm3_kernel(const float* __restrict__ A1s, const float* __restrict__ A2s,
+                                                       const float* __restrict__ A1d, const float* __restrict__ A2d,
+                                                       long long ld, long long n, int kc,
+                                                       const uint4* __restrict__ planes, long long plane_len,
+                                                       float* __restrict__ partial, long long npad,
+                                                       const MemberCtrl* __restrict__ mc) {
+    const unsigned long long act = __ballot(mc[threadIdx.x & 63].active != 0);
+    int L = 0;
+#pragma unroll
+    for (int j = 0; j < 4; ++j)
+        if (act & (0x1111111111111111ull << j)) L = j + 1;
+    if (L == 4) ens_gemm3_tile<4>(A1s, A2s, A1d, A2d, ld, n, kc, planes, plane_len, partial, npad);
+    else if (L == 3) ens_gemm3_tile<3>(A1s, A2s, A1d, A2d, ld, n, kc, planes, plane_len, partial, npad);
+    else if (L == 2) ens_gemm3_tile<2>(A1s, A2s, A1d, A2d, ld, n, kc, planes, plane_len, partial, npad);
+    else ens_gemm3_tile<1>(A1s, A2s, A1d, A2d, ld, n, kc, planes, plane_len, partial, npad);
 }
 
 // ---------------------------------------------------------------------------
@@ -506,6 +696,10 @@ struct Ensemble {
     int graph_steps = 0;
     int zero_mask = 0;    // bit k: matrix k holds only zeros (from the last scan)
     int skip_normal = 0;  // Bs = Bd = 0: the GEMM skips the normal component
+    // product arithmetic: 0 FP32 MFMA; 1 three-part BF16 split on the BF16
+    // MFMA units (used when skip_normal; the three-component GEMM stays FP32)
+    int prec = 0;
+    uint4* planes = nullptr;  // u's BF16 parts: 2 arrays x 3 parts x (npad / 8) x 64 x 16 bytes
 };
 
 static EnsArgs ens_args(Ensemble* e) {
@@ -531,15 +725,21 @@ static void ens_launch_prep(Ensemble* e, hipStream_t st) {
     hipLaunchKernelGGL(ens_prep_u_kernel, dim3((unsigned)((e->n + 63) / 64)), dim3(256), 0, st,
                        f + (size_t)E_SLIPV1 * items, f + (size_t)E_SLIPV2 * items, f + (size_t)E_COSR0 * items,
                        f + (size_t)E_SINR0 * items, e->Vpl, e->n, e->B, e->mc, e->ctrl, e->U,
-                       e->U + (size_t)e->n * e->B);
+                       e->U + (size_t)e->n * e->B, e->prec ? e->planes : nullptr, e->npad / 8 * 64);
 }
+
+// the BF16-split GEMM is in use
+static bool ens_split_active(const Ensemble* e) { return e->prec == 1 && e->skip_normal && e->kc % 32 == 0; }
 
 // the GEMM alone, on the operands of the last ens_launch_prep
 static void ens_launch_gemm_only(Ensemble* e, hipStream_t st) {
     const float* U1 = e->U;
     const float* U2 = e->U + (size_t)e->n * e->B;
     dim3 grid((unsigned)((e->n + 63) / 64), (unsigned)e->nchunk);
-    if (e->skip_normal)
+    if (ens_split_active(e))
+        hipLaunchKernelGGL(ens_gemm3_kernel, grid, dim3(64), 0, st, e->mats[0], e->mats[1], e->mats[3], e->mats[4],
+                           e->ld, e->n, e->kc, e->planes, e->npad / 8 * 64, e->partial, e->npad, e->mc);
+    else if (e->skip_normal)
         hipLaunchKernelGGL(ens_gemm_kernel<2>, grid, dim3(64), 0, st, e->mats[0], e->mats[1], e->mats[2], e->mats[3],
                            e->mats[4], e->mats[5], e->ld, e->n, e->kc, U1, U2, e->partial, e->npad, e->mc);
     else
@@ -659,6 +859,7 @@ int pq3d_ens_destroy(void* h) {
     for (int k = 0; k < 6; ++k) (void)hipFree(e->own_mats[k]);
     (void)hipFree(e->f);
     (void)hipFree(e->U);
+    (void)hipFree(e->planes);
     (void)hipFree(e->partial);
     (void)hipFree(e->mc);
     (void)hipFree(e->slot);
@@ -820,7 +1021,9 @@ int pq3d_ens_set_kc(void* h, int kc) {
     Ensemble* e = (Ensemble*)h;
     PQ_CHECK(e, "pq3d_ens_set_kc: null handle");
     PQ_CHECK(kc >= 4, "pq3d_ens_set_kc: kc must be >= 4 (got %d)", kc);
-    const long long kc4 = std::min<long long>(((long long)kc + 3) / 4 * 4, (e->n + 3) / 4 * 4);
+    const int step = e->prec == 1 ? 32 : 4;  // the BF16-split GEMM walks 32 sources per step
+    const long long kc4 = std::min<long long>(((long long)kc + step - 1) / step * step,
+                                              (e->n + step - 1) / step * step);
     const int nchunk = (int)((e->n + kc4 - 1) / kc4);
     PQ_HIP(hipStreamSynchronize(e->stream));
     ens_drop_graph(e);
@@ -832,6 +1035,37 @@ int pq3d_ens_set_kc(void* h, int kc) {
     e->partial = p;
     e->kc = (int)kc4;
     e->nchunk = nchunk;
+    return 0;
+}
+
+// Product arithmetic (no reference counterpart): 0 FP32 MFMA; 1 every FP32
+// operand split into three BF16 parts, six products on the BF16 MFMA units
+// with FP32 accumulation (the two-component GEMM of Bs = Bd = 0 only; the
+// three-component one stays FP32).  Mode 1 rounds kc up to a multiple of 32.
+int pq3d_ens_set_precision(void* h, int mode) {
+    clear_error();
+    Ensemble* e = (Ensemble*)h;
+    PQ_CHECK(e, "pq3d_ens_set_precision: null handle");
+    PQ_CHECK(mode == 0 || mode == 1, "pq3d_ens_set_precision: mode must be 0 or 1 (got %d)", mode);
+    PQ_HIP(hipStreamSynchronize(e->stream));
+    ens_drop_graph(e);
+    if (mode == 1 && !e->planes) {
+        const size_t bytes = (size_t)2 * 3 * (e->npad / 8) * 64 * sizeof(uint4);
+        PQ_HIP(hipMalloc((void**)&e->planes, bytes));
+        PQ_HIP(hipMemset(e->planes, 0, bytes));  // sources past n stay zero
+    }
+    e->prec = mode;
+    if (mode == 1 && e->kc % 32) return pq3d_ens_set_kc(h, e->kc);
+    return 0;
+}
+
+// mode as set, and whether the BF16-split GEMM is the one launched
+int pq3d_ens_get_precision(void* h, int* mode, int* split_active) {
+    clear_error();
+    Ensemble* e = (Ensemble*)h;
+    PQ_CHECK(e && mode && split_active, "pq3d_ens_get_precision: null argument");
+    *mode = e->prec;
+    *split_active = ens_split_active(e) ? 1 : 0;
     return 0;
 }
 

@@ -40,6 +40,8 @@ def _declare(L):
     _lib._sig(L, "pq3d_ens_time_product", [vp, i, ctypes.POINTER(ctypes.c_float)])
     _lib._sig(L, "pq3d_ens_set_kc", [vp, i])
     _lib._sig(L, "pq3d_ens_get_kc", [vp, ctypes.POINTER(i), ctypes.POINTER(i)])
+    _lib._sig(L, "pq3d_ens_set_precision", [vp, i])
+    _lib._sig(L, "pq3d_ens_get_precision", [vp, ctypes.POINTER(i), ctypes.POINTER(i)])
     _lib._sig(L, "pq3d_ens_scan_structure", [vp, ctypes.POINTER(i)])
     _lib._sig(L, "pq3d_ens_set_structure", [vp, i])
     _lib._sig(L, "pq3d_ens_get_structure", [vp, ctypes.POINTER(i), ctypes.POINTER(i)])
@@ -52,7 +54,7 @@ _lib.register(_declare)
 
 
 class Ensemble:
-    def __init__(self, n, B=64, matrices_from=None, kc=0):
+    def __init__(self, n, B=64, matrices_from=None, kc=0, precision="bf16x3"):
         L = _lib.lib()
         h = ctypes.c_void_p()
         if matrices_from is not None:
@@ -76,6 +78,8 @@ class Ensemble:
         self.exploit_structure = True
         self.mat_gen = 0
         self._structure_gen = None
+        if precision != "fp32":
+            self.set_precision(precision)
 
     def matrices_changed(self):
         """Mark owned matrices as rewritten through ``mat_ptrs``."""
@@ -162,6 +166,24 @@ class Ensemble:
 
     def set_kc(self, kc):
         check(_lib.lib().pq3d_ens_set_kc(self.h, int(kc)), "ens set kc")
+
+    PRECISIONS = {"fp32": 0, "bf16x3": 1}
+
+    def set_precision(self, mode):
+        """GEMM arithmetic: "fp32" (FP32 MFMA) or "bf16x3" (each FP32 operand
+        split exactly into three BF16 parts, the six leading products on the
+        BF16 MFMA units with FP32 accumulation; used for the two-component
+        GEMM of Bs = Bd = 0, the three-component one stays FP32).  "bf16x3"
+        rounds kc up to a multiple of 32."""
+        check(_lib.lib().pq3d_ens_set_precision(self.h, int(self.PRECISIONS[mode])), "ens set precision")
+
+    @property
+    def precision(self):
+        """(mode as set, whether the BF16-split GEMM is the one launched)."""
+        self._sync_structure()
+        m, a = ctypes.c_int(), ctypes.c_int()
+        check(_lib.lib().pq3d_ens_get_precision(self.h, ctypes.byref(m), ctypes.byref(a)), "ens get precision")
+        return ("fp32", "bf16x3")[m.value], bool(a.value)
 
     def gemm_result(self):
         out = np.empty((3, self.B, self.n))

@@ -52,7 +52,7 @@ def test_gemm_any_width_matches_fp64(mesh, tmp_path, monkeypatch):
     n = sim.n
     mats = [sim.download_matrix(k) for k in range(6)]
     xg = mesh[3]
-    e = Ensemble(n, B, matrices_from=sim)
+    e = Ensemble(n, B, matrices_from=sim, precision="fp32")
     init_members(e, np.stack([synth.member_params(xg, m) for m in range(B)]), V0, VPL, MU, CS)
     rng = np.random.default_rng(9)
     V1 = rng.lognormal(-18, 3, (B, n)) * rng.choice([-1, 1], (B, n))
@@ -84,6 +84,52 @@ def test_gemm_any_width_matches_fp64(mesh, tmp_path, monkeypatch):
     got = e.gemm_result()
     full = (n + 3) // 4 * 4
     assert (np.abs(got - results[full]) / mag).max() < (e.kc + full + 16) * u
+    e.close()
+    sim.close()
+
+
+def test_split_gemm_any_width_matches_fp64(mesh, tmp_path, monkeypatch):
+    """The BF16-split GEMM (three exact BF16 parts per FP32 operand, six
+    products, 32-source steps) at widths 32, 128, n / 3 and n, against the
+    FP64 product of the same FP32 operands: the FP32 GEMM's bars, (kc + 8) u
+    per entry and sqrt(kc) u root-mean-square of sum_j |a_ij u_j|; the
+    tuner's widths are multiples of 32 and cached under their own key."""
+    from pyquake3d_amd import _lib, autotune, synth
+    from pyquake3d_amd.ensemble import Ensemble, init_members
+    monkeypatch.setenv("PQ3D_TUNE_CACHE", str(tmp_path / "t.json"))
+    sim = _sim(mesh, _lib.MAT_F32)
+    n = sim.n
+    mats = [sim.download_matrix(k) for k in range(6)]
+    xg = mesh[3]
+    e = Ensemble(n, B, matrices_from=sim)
+    assert e.precision == ("bf16x3", True)
+    init_members(e, np.stack([synth.member_params(xg, m) for m in range(B)]), V0, VPL, MU, CS)
+    rng = np.random.default_rng(19)
+    V1 = rng.lognormal(-18, 3, (B, n)) * rng.choice([-1, 1], (B, n))
+    V2 = rng.lognormal(-18, 3, (B, n))
+    e.set("slipv1", V1)
+    e.set("slipv2", V2)
+    r0 = e.get("rake0")
+    U1 = (V1 - VPL * np.cos(r0)).astype(np.float32).astype(np.float64)
+    U2 = (V2 - VPL * np.sin(r0)).astype(np.float32).astype(np.float64)
+    u = 2.0 ** -24
+    want = np.stack([U1 @ mats[q].T + U2 @ mats[q + 3].T for q in range(2)])
+    mag = np.stack([np.abs(U1) @ np.abs(mats[q]).T + np.abs(U2) @ np.abs(mats[q + 3]).T for q in range(2)])
+    mag = np.where(mag > 0, mag, 1.0)
+    for kc in (32, 100, n // 3, n):
+        e.set_kc(kc)
+        assert e.kc % 32 == 0 and e.kc >= min(kc, n) and e.kc < min(kc, n) + 32, (kc, e.kc)
+        e.gemm()
+        got = e.gemm_result()
+        assert not np.any(got[2])   # Bs = Bd = 0: the normal component is exactly zero
+        err = np.abs(got[:2] - want) / mag
+        assert err.max() < (e.kc + 8) * u, (kc, err.max())
+        assert np.sqrt(np.mean(err ** 2)) < np.sqrt(e.kc) * u, (kc, np.sqrt(np.mean(err ** 2)))
+    res = autotune.tune_ensemble(e)
+    assert res["kc"] % 32 == 0 and all(k % 32 == 0 for k, _ in res["table"])
+    assert res["key"].endswith("|bf16x3") and autotune.cached(res["key"]) == res["kc"]
+    e.set_precision("fp32")
+    assert e.precision == ("fp32", False) and autotune.ensemble_key(e) != res["key"]
     e.close()
     sim.close()
 
@@ -156,8 +202,10 @@ def test_zero_normal_matrices_skipped_bit_identically(mesh):
         assert np.array_equal(a.get(name), b.get(name)), name
     xg = mesh[3]
     cols = np.stack([synth.member_params(xg, m) for m in range(B)])
-    e1 = Ensemble(a.n, B, matrices_from=a)
-    e2 = Ensemble(a.n, B, matrices_from=a)
+    # FP32 products on both sides (the BF16-split GEMM serves only the
+    # two-component case, so it has no six-matrix twin to compare with)
+    e1 = Ensemble(a.n, B, matrices_from=a, precision="fp32")
+    e2 = Ensemble(a.n, B, matrices_from=a, precision="fp32")
     e2.exploit_structure = False
     for e in (e1, e2):
         init_members(e, cols, V0, VPL, MU, CS)

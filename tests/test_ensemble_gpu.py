@@ -1,9 +1,14 @@
 """GPU tests of the ensemble mode (64 members on one mesh, MFMA GEMM).
 
+Every test of the GEMM and the trajectories runs with both products: FP32
+inputs on the FP32 MFMA units, and the same FP32 inputs split exactly into
+three BF16 parts each with the six leading products on the BF16 MFMA units
+(the Python default on a planar fault).
+
 Bars:
-* GEMM (FP32 inputs on MFMA, FP32 accumulation, split-K partials summed in
-  FP64): per row, error <= 2e-6 of sum_j |a_ij u_j| against an FP64 product of
-  the same FP32 matrices;
+* GEMM (FP32 accumulation, split-K partials summed in FP64): per row, error
+  <= 2e-6 of sum_j |a_ij u_j| against an FP64 product of the same FP32
+  matrices;
 * member trajectories: each member against the NumPy FP64 oracle with that
   member's parameters and the same FP32-rounded matrices, 30 global steps:
   step sizes within 1e-6 and states within 1e-6 of their scale;
@@ -37,17 +42,22 @@ def mesh():
     return sim, mats, cols, xg
 
 
-def _ens(mesh):
+PRECISIONS = ("fp32", "bf16x3")
+
+
+def _ens(mesh, precision="bf16x3"):
     from pyquake3d_amd.ensemble import Ensemble, init_members
     sim, mats, cols, xg = mesh
-    e = Ensemble(sim.n, B, matrices_from=sim, kc=128)
+    e = Ensemble(sim.n, B, matrices_from=sim, kc=128, precision=precision)
+    assert e.precision == (precision, precision == "bf16x3")  # planar mesh: Bs = Bd = 0
     init_members(e, cols, V0, VPL, MU, CS)
     return e
 
 
-def test_gemm_matches_fp64_product(mesh):
+@pytest.mark.parametrize("precision", PRECISIONS)
+def test_gemm_matches_fp64_product(mesh, precision):
     sim, mats, cols, xg = mesh
-    e = _ens(mesh)
+    e = _ens(mesh, precision)
     rng = np.random.default_rng(4)
     n = sim.n
     V1 = rng.lognormal(-18, 3, (B, n)) * rng.choice([-1, 1], (B, n))
@@ -67,9 +77,10 @@ def test_gemm_matches_fp64_product(mesh):
     e.close()
 
 
-def test_members_follow_the_oracle(mesh):
+@pytest.mark.parametrize("precision", PRECISIONS)
+def test_members_follow_the_oracle(mesh, precision):
     sim, mats, cols, xg = mesh
-    e = _ens(mesh)
+    e = _ens(mesh, precision)
     K = 30
     e.run(K, graph_steps=4)
     mem = e.members()
@@ -111,13 +122,14 @@ def test_failed_member_is_masked_and_others_unchanged(mesh):
     b.close()
 
 
-def test_gemm_skips_idle_member_subtiles_exactly(mesh):
+@pytest.mark.parametrize("precision", PRECISIONS)
+def test_gemm_skips_idle_member_subtiles_exactly(mesh, precision):
     """Member subtile j holds members j, j + 4, ..., j + 60; the GEMM skips the
     subtiles above the highest one with an active member.  Active members'
     results must be bit-identical to the all-active product and inactive
     members' exactly zero, whichever subtiles are live."""
     sim, mats, cols, xg = mesh
-    e = _ens(mesh)
+    e = _ens(mesh, precision)
     rng = np.random.default_rng(8)
     n = sim.n
     e.set("slipv1", rng.lognormal(-18, 3, (B, n)) * rng.choice([-1, 1], (B, n)))
