@@ -82,9 +82,16 @@ def _nq(obj):
 
 
 def _split(ens):
-    """The ensemble runs the BF16-split GEMM (32-source steps)."""
+    """The ensemble runs the BF16-split GEMM."""
     p = getattr(ens, "precision", None)
     return bool(p[1]) if p is not None else False
+
+
+def _kc_step(ens):
+    """Widths are multiples of 32 once the BF16 split is requested (whether or
+    not it runs), else of 4."""
+    p = getattr(ens, "precision", None)
+    return 32 if (p is not None and p[0] != "fp32") else 4
 
 
 def ensemble_key(ens):
@@ -137,8 +144,7 @@ def tune_ensemble(ens, candidates=None, reps=4, trials=2, use_cache=True):
     """Measure split-K widths for an ``Ensemble`` (its matrices resident),
     apply the fastest and cache it.  Returns {"kc", "ms", "table", "key"}."""
     key = ensemble_key(ens)
-    cands = (list(candidates) if candidates is not None
-             else ensemble_candidates(ens.n, 32 if _split(ens) else 4))
+    cands = list(candidates) if candidates is not None else ensemble_candidates(ens.n, _kc_step(ens))
     (kc, ms), table = _sweep(ens, cands, reps, trials)
     entry = {"kc": kc, "ms": ms, "candidates": len(table)}
     if use_cache:

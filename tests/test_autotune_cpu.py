@@ -90,3 +90,37 @@ def test_switch_off(monkeypatch):
     assert not autotune.enabled()
     monkeypatch.setenv("PQ3D_AUTOTUNE", "1")
     assert autotune.enabled()
+
+
+class FakeSplit(FakeTiled):
+    """An ensemble with the BF16 split requested: widths round to 32; the
+    split runs only when `active` (Bs = Bd = 0)."""
+
+    def __init__(self, n, best, active):
+        super().__init__(n, best)
+        self._active = active
+
+    def set_kc(self, kc):
+        self._kc = (kc + 31) // 32 * 32
+
+    @property
+    def precision(self):
+        return "bf16x3", self._active
+
+    @property
+    def structure(self):
+        return (0b100100, True) if self._active else (0, False)
+
+
+def test_split_widths_and_cache_keys(cache):
+    on, off = FakeSplit(20000, 3360, True), FakeSplit(20000, 3360, False)
+    c = autotune.ensemble_candidates(20000, 32)
+    assert all(k % 32 == 0 for k in c) and len(set(c)) == len(c)
+    r_on, r_off = autotune.tune_ensemble(on), autotune.tune_ensemble(off)
+    # both sweep 32-multiples (no duplicate candidates after rounding) ...
+    assert all(k % 32 == 0 for k, _ in r_on["table"]) and len({k for k, _ in r_on["table"]}) == len(r_on["table"])
+    assert all(k % 32 == 0 for k, _ in r_off["table"]) and len({k for k, _ in r_off["table"]}) == len(r_off["table"])
+    # ... but only the running split GEMM gets its own cache entry
+    assert r_on["key"].endswith("|nq2|bf16x3") and r_off["key"].endswith("|nq3")
+    d = json.loads(cache.read_text())
+    assert d[r_on["key"]]["kc"] == r_on["kc"] and d[r_off["key"]]["kc"] == r_off["kc"]
