@@ -139,6 +139,8 @@ __global__ __launch_bounds__(256) void gf_segments_kernel(const SrcGeom<2>* __re
 
 static inline int popc(int x) { return __builtin_popcount((unsigned)x); }
 
+static int d2h_staged(void* dst, const void* src, size_t bytes);
+
 // Evaluate `segs` (descriptors with pair_off/out_off filled) whose index
 // lists are already on the device; results copied into host `out`.
 static int run_segments(GFSource* g, const int* d_ridx, const int* d_sidx, std::vector<SegDesc>& segs,
@@ -158,7 +160,7 @@ static int run_segments(GFSource* g, const int* d_ridx, const int* d_sidx, std::
                        g->halfspace, g->out);
     PQ_HIP(hipGetLastError());
     PQ_HIP(hipEventRecord(b, 0));
-    PQ_HIP(hipMemcpy(host_out, g->out, (size_t)total_out * sizeof(double), hipMemcpyDeviceToHost));
+    if (d2h_staged(host_out, g->out, (size_t)total_out * sizeof(double))) return -1;
     float ms = 0;
     PQ_HIP(hipEventElapsedTime(&ms, a, b));
     if (gpu_ms) *gpu_ms += ms;
@@ -493,6 +495,55 @@ static void parallel_for(long long n, F f) {
             for (long long i = t; i < n; i += nt) f(i);
         });
     for (auto& x : th) x.join();
+}
+
+// Device -> pageable host copy through two pinned 64 MB buffers: the PCIe
+// transfer of one chunk overlaps the host threads' copy (and first-touch page
+// faults) of the previous one.  Copies of one chunk or less go direct.
+static int d2h_staged(void* dst, const void* src, size_t bytes) {
+    const size_t CH = (size_t)64 << 20;
+    if (bytes <= CH) {
+        PQ_HIP(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost));
+        return 0;
+    }
+    void* pin[2] = {nullptr, nullptr};
+    hipEvent_t ev[2] = {nullptr, nullptr};
+    hipError_t e = hipHostMalloc(&pin[0], CH, hipHostMallocDefault);
+    if (e == hipSuccess) e = hipHostMalloc(&pin[1], CH, hipHostMallocDefault);
+    if (e == hipSuccess) e = hipEventCreate(&ev[0]);
+    if (e == hipSuccess) e = hipEventCreate(&ev[1]);
+    const size_t nch = (bytes + CH - 1) / CH;
+    auto len = [&](size_t c) { return std::min(CH, bytes - c * CH); };
+    auto host_copy = [&](size_t c) {
+        char* d = (char*)dst + c * CH;
+        const char* sp = (const char*)pin[c & 1];
+        const size_t L = len(c), parts = 64, per = (L + parts - 1) / parts;
+        parallel_for((long long)parts, [&](long long q) {
+            const size_t o = (size_t)q * per;
+            if (o < L) memcpy(d + o, sp + o, std::min(per, L - o));
+        });
+    };
+    for (size_t c = 0; c < nch && e == hipSuccess; ++c) {
+        e = hipMemcpyAsync(pin[c & 1], (const char*)src + c * CH, len(c), hipMemcpyDeviceToHost, 0);
+        if (e == hipSuccess) e = hipEventRecord(ev[c & 1], 0);
+        if (e == hipSuccess && c >= 1) {
+            e = hipEventSynchronize(ev[(c - 1) & 1]);
+            if (e == hipSuccess) host_copy(c - 1);
+        }
+    }
+    if (e == hipSuccess) {
+        e = hipEventSynchronize(ev[(nch - 1) & 1]);
+        if (e == hipSuccess) host_copy(nch - 1);
+    }
+    for (int k = 0; k < 2; ++k) {
+        if (ev[k]) (void)hipEventDestroy(ev[k]);
+        if (pin[k]) (void)hipHostFree(pin[k]);
+    }
+    if (e != hipSuccess) {
+        set_error("device to host copy: %s", hipGetErrorString(e));
+        return -1;
+    }
+    return 0;
 }
 
 // ---------------------------------------------------------------------------
@@ -2283,8 +2334,8 @@ int pq3d_aca_fetch(void* r, double* U, double* V) {
     size_t uo = 0, vo = 0;
     if (res->dU) {  // device result: one copy each, then the instances redone on the host
         const auto t0 = std::chrono::steady_clock::now();
-        if (res->usz) PQ_HIP(hipMemcpy(U, res->dU, (size_t)res->usz * sizeof(double), hipMemcpyDeviceToHost));
-        if (res->vsz) PQ_HIP(hipMemcpy(V, res->dV, (size_t)res->vsz * sizeof(double), hipMemcpyDeviceToHost));
+        if (res->usz && d2h_staged(U, res->dU, (size_t)res->usz * sizeof(double))) return -1;
+        if (res->vsz && d2h_staged(V, res->dV, (size_t)res->vsz * sizeof(double))) return -1;
         for (size_t x = 0; x < res->inst.size(); ++x) {
             const Inst& s = res->inst[x];
             const size_t us = (size_t)s.k * s.m, vs = (size_t)s.k * s.n;
