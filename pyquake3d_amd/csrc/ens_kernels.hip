@@ -16,6 +16,9 @@
 //      source-major matrices (row = 4*(l%16) + subtile) and the four 16-member
 //      subtiles from one 16-byte load of u (member = 4*(l%16) + subtile).
 //      Split-K partials per column chunk are summed in a fixed order.
+//      ens_gemm3_kernel (precision mode 1, when Bs = Bd = 0) is the same tile
+//      with every FP32 operand split exactly into three BF16 parts and the
+//      six leading part products on v_mfma_f32_16x16x32_bf16.
 //   3. R rounds of
 //        ens_attempt_kernel: one Dormand-Prince attempt for every (member,
 //          element) item of members still pending in this step; per-member
