@@ -6,9 +6,11 @@
 // (ACA_plus_full_traceS/D) with its row/column evaluators :329-658, and the
 // dense leaves of ACA_worker :1292-1344.  The reference evaluates one row or
 // column per ctypes call and pivots one block at a time; here every sweep
-// evaluates the pending pivot rows/columns of ALL blocks in one launch
-// (``gf_segments_kernel``), and the pivot logic of all blocks advances in
-// lock step on host threads.
+// evaluates the pending pivot rows/columns of ALL blocks in one launch, and
+// the pivot logic of all blocks advances in lock step: on host threads
+// (pq3d_aca_run, evaluation by ``gf_segments_kernel``), or on the device, one
+// wave per (block, matrix), with the recompression there too
+// (pq3d_aca_run_device, the ``aca_dev_*`` kernels).
 //
 // ACA per (block, matrix): partial pivoting with a reference row and column
 // (ACA+): the next pivot is led by whichever of the reference row/column
