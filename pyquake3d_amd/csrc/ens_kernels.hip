@@ -19,7 +19,9 @@
 //      ens_gemm3_kernel (precision mode 1, when Bs = Bd = 0) is the same tile
 //      with every FP32 operand split exactly into three BF16 parts and the
 //      six leading part products on v_mfma_f32_16x16x32_bf16.
-//   3. R rounds of
+//   3. ens_rates_kernel:   the partials summed in chunk order into the active
+//      members' frozen traction rates.
+//   4. R rounds of
 //        ens_attempt_kernel: one Dormand-Prince attempt for every (member,
 //          element) item of members still pending in this step; per-member
 //          error maxima by wave shuffles and one atomic per wave;
@@ -101,6 +103,7 @@ struct EnsArgs {
     long long n, npad;
     int B, nchunk;
     double V0, Vpl, eta;
+    int rates_ready;           // ens_rates_kernel has filled the frozen traction rates
 };
 
 // ---------------------------------------------------------------------------
@@ -538,6 +541,29 @@ __device__ __forceinline__ void member_max_atomic(int m, bool valid, unsigned lo
 
 __device__ __forceinline__ bool pending(const MemberCtrl& c) { return c.active && !c.accepted; }
 
+// The frozen traction rates of every active member from the GEMM's split-K
+// partials, summed in chunk order in FP64 (what the first attempt round does
+// otherwise, one item per lane; as a pass of its own the partial reads
+// stream at full rate instead of waiting inside the RK arithmetic).
+__global__ __launch_bounds__(256) void ens_rates_kernel(EnsArgs A) {
+    const long long n = A.n, items = n * A.B;
+    const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= items) return;
+    const int m = (int)(t / n);
+    if (!A.mc[m].active) return;  // round 0 reads A.mc's first half, as here
+    const long long e = t - (long long)m * n;
+    double s0 = 0, s1 = 0, s2 = 0;
+    for (int c = 0; c < A.nchunk; ++c) {
+        s0 += A.partial[(((size_t)c * 3 + 0) * A.B + m) * A.npad + e];
+        s1 += A.partial[(((size_t)c * 3 + 1) * A.B + m) * A.npad + e];
+        s2 += A.partial[(((size_t)c * 3 + 2) * A.B + m) * A.npad + e];
+    }
+    double* f = A.f + t;
+    f[(size_t)E_ADOTV1 * items] = s0;
+    f[(size_t)E_ADOTV2 * items] = s1;
+    f[(size_t)E_DSIGMADT * items] = s2 + f[(size_t)E_NORMAL_LOADING * items];
+}
+
 // One attempt for every item of a pending member; round 0 first sums the
 // GEMM's split-K partials into the frozen traction rates.
 __global__ __launch_bounds__(256) void ens_attempt_kernel(EnsArgs A, int round) {
@@ -557,7 +583,7 @@ __global__ __launch_bounds__(256) void ens_attempt_kernel(EnsArgs A, int round) 
     auto F = [&](int field) -> double& { return f[(size_t)field * items + tt]; };
     unsigned long long r1b = 0ull, r2b = 0ull;
     if (pend) {
-        if (round == 0) {
+        if (round == 0 && !A.rates_ready) {
             const long long e = tt - (long long)m * n;
             double s0 = 0, s1 = 0, s2 = 0;
             for (int c = 0; c < A.nchunk; ++c) {
@@ -719,6 +745,7 @@ static EnsArgs ens_args(Ensemble* e) {
     a.V0 = e->V0;
     a.Vpl = e->Vpl;
     a.eta = e->eta;
+    a.rates_ready = 0;
     return a;
 }
 
@@ -757,8 +784,10 @@ static void ens_launch_gemm(Ensemble* e, hipStream_t st) {
 
 static void ens_launch_step(Ensemble* e, hipStream_t st) {
     ens_launch_gemm(e, st);
-    const EnsArgs a = ens_args(e);
+    EnsArgs a = ens_args(e);
     const unsigned blocks = (unsigned)((e->n * e->B + 255) / 256);
+    hipLaunchKernelGGL(ens_rates_kernel, dim3(blocks), dim3(256), 0, st, a);
+    a.rates_ready = 1;
     for (int r = 0; r < ENS_ROUNDS; ++r) {
         hipLaunchKernelGGL(ens_attempt_kernel, dim3(blocks), dim3(256), 0, st, a, r);
         hipLaunchKernelGGL(ens_decide_kernel, dim3(blocks), dim3(256), 0, st, a, r);
