@@ -240,7 +240,7 @@ int pq3d_aca_run(void* src, long long nb, const long long* row_off, const int* r
  * of every evaluated row and column, and the recompression (same algorithm)
  * on the device too; the final factors stay in HBM until pq3d_aca_fetch
  * copies them out.  Factors of at most kcap terms live on the device
- * (kcap <= 0: 48; at most 64); an instance needing more is redone and
+ * (kcap <= 0: 64, the most); an instance needing more is redone and
  * recompressed by the host loop from its own seed.  Sums run in another order
  * than the host loop's, so pivots may differ where residual entries nearly
  * tie; the stopping rules and their tolerances are the same.  Results as

@@ -23,10 +23,12 @@ Layers, each mirroring the reference's names and argument meaning:
 
 Where the work runs: the matrix entries of every ACA pivot row/column and of
 every dense leaf are evaluated on the GPU (one batched launch per ACA sweep
-over all blocks, ``pq3d_gf_segments``); the ACA pivoting runs natively on the
-host (``pq3d_aca_run``); ``DeviceHmatrix`` packs the factors and dense leaves
-into a device-resident, cluster-ordered layout and runs the product with HIP
-kernels (``pq3d_hmat_*``), which is also what the stepping path uses.
+over all blocks, ``pq3d_gf_segments``); the ACA pivoting, residual updates
+and recompression run on the GPU too (``pq3d_aca_run_device``, the default;
+``pq3d_aca_run`` keeps the pivoting on host threads); ``DeviceHmatrix`` packs
+the factors and dense leaves into a device-resident, cluster-ordered layout
+and runs the product with HIP kernels (``pq3d_hmat_*``), which is also what
+the stepping path uses.
 
 ACA stopping rule: the reference stops on an absolute step size of 0.01 (in
 Pa/m of slip, :1301-1321) or a pivot below 1e-4; here the rule is relative,
@@ -37,6 +39,7 @@ reference's (it draws its start rows from an unseeded RNG, :740), so factors
 match the reference only through the accuracy bound.
 """
 import ctypes
+import math
 import time
 
 import numpy as np
@@ -45,6 +48,8 @@ from . import _lib, readmsh
 from ._lib import check, ptr
 
 MATRIX_NAMES = ("A1s", "A2s", "Bs", "A1d", "A2d", "Bd")
+_BLOCK_LISTS = frozenset([p + s for p in ("U_", "S_", "Vt_") for s in ("1s", "2s", "Bs", "1d", "2d", "Bd")]
+                         + ["Mf_" + n for n in MATRIX_NAMES])
 
 
 # ----------------------------------------------------------------- geometry
@@ -55,18 +60,44 @@ def bounding_box(cluster, points):
 
 
 def diameter(a, b):
-    """Diagonal length of the box [a, b]."""
-    return float(np.sqrt(np.sum((b - a) ** 2)))
+    """Diagonal length of the box [a, b].  Python floats, summed over the
+    axes in order: the same roundings as NumPy's sum of three squares, without
+    its per-call overhead (the block tree evaluates this ~10^5 times)."""
+    s = 0.0
+    for x, y in zip(np.asarray(a, dtype=np.float64).tolist(), np.asarray(b, dtype=np.float64).tolist()):
+        d = y - x
+        s += d * d
+    return math.sqrt(s)
 
 
 def distance(a_tau, b_tau, a_sigma, b_sigma):
     """Euclidean distance between the boxes [a_tau, b_tau] and [a_sigma, b_sigma]."""
-    gap = np.maximum(0.0, a_sigma - b_tau) ** 2 + np.maximum(0.0, a_tau - b_sigma) ** 2
-    return float(np.sqrt(np.sum(gap)))
+    return _box_distance(*(np.asarray(v, dtype=np.float64).tolist() for v in (a_tau, b_tau, a_sigma, b_sigma)))
+
+
+def _box_distance(a_tau, b_tau, a_sigma, b_sigma):
+    """``distance`` on lists of floats: per axis max(0, a_s - b_t)^2 +
+    max(0, a_t - b_s)^2, summed over the axes in order."""
+    s = 0.0
+    for at, bt, as_, bs in zip(a_tau, b_tau, a_sigma, b_sigma):
+        g1 = as_ - bt
+        g2 = at - bs
+        g1 = g1 if g1 > 0.0 else 0.0
+        g2 = g2 if g2 > 0.0 else 0.0
+        s += g1 * g1 + g2 * g2
+    return math.sqrt(s)
 
 
 def _admissible_boxes(lo_t, hi_t, lo_s, hi_s, eta):
     return min(diameter(lo_t, hi_t), diameter(lo_s, hi_s)) <= eta * distance(lo_t, hi_t, lo_s, hi_s)
+
+
+def _admissible_nodes(t, s, eta):
+    """``_admissible_boxes`` for two cluster-tree nodes, from their cached
+    corners and diameters."""
+    lo_t, hi_t, d_t = t.geometry()
+    lo_s, hi_s, d_s = s.geometry()
+    return min(d_t, d_s) <= eta * _box_distance(lo_t, hi_t, lo_s, hi_s)
 
 
 def is_admissible(cluster_tau, cluster_sigma, points, eta=2.0):
@@ -94,6 +125,16 @@ class TreeNode:
         self.lo = None
         self.hi = None
         self.start = 0
+        self._geometry = None
+
+    def geometry(self):
+        """(lo, hi, diameter) of the bounding box, corners as lists of floats;
+        computed once per node."""
+        g = self._geometry
+        if g is None:
+            lo, hi = self.lo.tolist(), self.hi.tolist()
+            g = self._geometry = (lo, hi, diameter(lo, hi))
+        return g
 
 
 def build_block_tree(cluster, points, min_size=16, depth=0):
@@ -158,14 +199,8 @@ class Block:
         self.col_cluster = col_cluster
         self.row_index = row_index
         self.col_index = col_index
-        for s in ("1s", "2s", "Bs", "1d", "2d", "Bd"):
-            setattr(self, "U_" + s, [])
-            setattr(self, "S_" + s, [])
-            setattr(self, "Vt_" + s, [])
         self.ACA_dictS = {}
         self.ACA_dictD = {}
-        for name in MATRIX_NAMES:
-            setattr(self, "Mf_" + name, [])
         self.children = children if children is not None else []
         self.level = level
         self.judproc = False
@@ -173,6 +208,16 @@ class Block:
         self.judaca = False
         self.row_node = None
         self.col_node = None
+
+    def __getattr__(self, name):
+        # The per-matrix lists U_*, S_*, Vt_* and Mf_* start empty; each is
+        # made on first access, so the ~4*10^4 blocks of a large tree do not
+        # allocate 24 lists apiece up front.
+        if name in _BLOCK_LISTS:
+            v = []
+            setattr(self, name, v)
+            return v
+        raise AttributeError(name)
 
     def setdata(self, data1):
         self.data = data1
@@ -195,7 +240,7 @@ def create_recursive_blocks(row_cluster, col_cluster, row_index, col_index, poin
         b.row_node, b.col_node = row_cluster, col_cluster
         return b
 
-    if (_admissible_boxes(row_cluster.lo, row_cluster.hi, col_cluster.lo, col_cluster.hi, eta)
+    if (_admissible_nodes(row_cluster, col_cluster, eta)
             or len(row_cluster.indices) <= mini_leaf or len(col_cluster.indices) <= mini_leaf):
         return leaf()
     if (row_cluster.left is None or row_cluster.right is None or col_cluster.left is None
@@ -344,7 +389,8 @@ def aca_blocks(src, blocks, eps=1e-4, recompress_eps=None, max_rank=0, seed=1234
 
     device=True runs the pivot logic on the GPU (``pq3d_aca_run_device``, one
     wave per block and matrix, factors of up to ``kcap`` terms on the device,
-    default 48); a matrix needing more terms is redone by the host loop."""
+    default and at most 64); a matrix needing more terms is redone by the host
+    loop."""
     rc_eps = eps if recompress_eps is None else recompress_eps
     nb = len(blocks)
     rows = [_i32(b[0]) for b in blocks]

@@ -1970,7 +1970,7 @@ int pq3d_aca_run(void* h, long long nb, const long long* row_off, const int* row
 
 // The same ACA with the pivoting on the device (one wave per block and
 // matrix; no per-sweep transfer of values).  Factors of at most kcap terms
-// live on the device (kcap <= 0: 48, at most 64); an instance that needs more is redone
+// live on the device (kcap <= 0: 64, the most); an instance that needs more is redone
 // by the host loop from its own seed.  Results, ranks and statistics as
 // pq3d_aca_run (pq3d_aca_fetch/stats/phases/free apply).
 int pq3d_aca_run_device(void* h, long long nb, const long long* row_off, const int* row_idx,
@@ -1979,7 +1979,7 @@ int pq3d_aca_run_device(void* h, long long nb, const long long* row_off, const i
     clear_error();
     GFSource* g = (GFSource*)h;
     if (int rc = aca_check(g, nb, row_off, row_idx, col_off, col_idx, eps)) return rc;
-    if (kcap <= 0) kcap = 48;
+    if (kcap <= 0) kcap = 64;
     PQ_CHECK(kcap <= 64, "pq3d_aca_run_device: kcap %d above 64 (one SVD column per lane)", kcap);
     if (nb == 0)  // nothing to launch: the host loop's empty result
         return pq3d_aca_run(h, nb, row_off, row_idx, col_off, col_idx, eps, recompress_eps, max_rank, seed, ranks,
