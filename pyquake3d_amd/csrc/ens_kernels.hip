@@ -356,14 +356,24 @@ __device__ __forceinline__ void ens_gemm3_tile(const float* __restrict__ A1s, co
             for (int j = 0; j < 4; ++j) acc[q][i][j] = v4f{0.f, 0.f, 0.f, 0.f};
     float4 as[2][8], ad[2][8];
     bf8 us[4][3], ud[4][3];
-    // sources past n read column n - 1 (their u parts are zero)
+    // A at (source kb + min(8 g + t, n - 1 - kb), row): sources past n read
+    // column n - 1 (their u parts are zero).  The address is the step's
+    // wave-uniform column base plus a 32-bit per-lane byte offset (ld < 2^25,
+    // checked on the host); the empty asm keeps the offset a 32-bit value, so
+    // the loads take the scalar-base form instead of sixteen per-lane 64-bit
+    // pointers advanced every step (bit-identical, 2 % faster at 20k).
+    const unsigned row4 = (unsigned)(row * 4), ld4 = (unsigned)(ld * 4);
     auto load_a = [&](float4 (&a)[2][8], const float* M0, const float* M1, long long kb) {
+        const int lim = (int)(n - 1 - kb < 31 ? n - 1 - kb : 31);
+        const char* B0 = reinterpret_cast<const char*>(M0 + (size_t)kb * ld);
+        const char* B1 = reinterpret_cast<const char*>(M1 + (size_t)kb * ld);
 #pragma unroll
         for (int t = 0; t < 8; ++t) {
-            long long k = kb + 8 * g + t;
-            k = k < n ? k : n - 1;
-            a[0][t] = *reinterpret_cast<const float4*>(M0 + (size_t)k * ld + row);
-            a[1][t] = *reinterpret_cast<const float4*>(M1 + (size_t)k * ld + row);
+            const int kk = 8 * g + t < lim ? 8 * g + t : lim;
+            unsigned o = (unsigned)kk * ld4 + row4;
+            asm volatile("" : "+v"(o));
+            a[0][t] = *reinterpret_cast<const float4*>(B0 + o);
+            a[1][t] = *reinterpret_cast<const float4*>(B1 + o);
         }
     };
     auto load_u = [&](bf8 (&uf)[4][3], int arr, long long kb) {
@@ -759,7 +769,10 @@ static void ens_launch_prep(Ensemble* e, hipStream_t st) {
 }
 
 // the BF16-split GEMM is in use
-static bool ens_split_active(const Ensemble* e) { return e->prec == 1 && e->skip_normal && e->kc % 32 == 0; }
+// (its matrix loads use 32-bit byte offsets within 32 columns: ld < 2^25)
+static bool ens_split_active(const Ensemble* e) {
+    return e->prec == 1 && e->skip_normal && e->kc % 32 == 0 && e->ld < (1LL << 25);
+}
 
 // the GEMM alone, on the operands of the last ens_launch_prep
 static void ens_launch_gemm_only(Ensemble* e, hipStream_t st) {
