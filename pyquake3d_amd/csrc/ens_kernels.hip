@@ -376,13 +376,19 @@ __device__ __forceinline__ void ens_gemm3_tile(const float* __restrict__ A1s, co
             a[1][t] = *reinterpret_cast<const float4*>(B1 + o);
         }
     };
+    // the members' parts: the step's uniform plane base plus a per-lane byte
+    // offset below 4 KB (member subtile j in the immediate offset)
+    const unsigned ulane = (unsigned)((g * 64 + 4 * lr) * 16);
     auto load_u = [&](bf8 (&uf)[4][3], int arr, long long kb) {
-        const uint4* base = planes + (size_t)arr * 3 * plane_len + ((size_t)(kb / 8 + g) * 64 + 4 * lr);
+        const uint4* base = planes + (size_t)arr * 3 * plane_len + (size_t)(kb / 8) * 64;
+        unsigned o = ulane;
+        asm volatile("" : "+v"(o));
 #pragma unroll
         for (int j = 0; j < L; ++j)
 #pragma unroll
             for (int p = 0; p < 3; ++p) {
-                const uint4 v = base[(size_t)p * plane_len + j];
+                const char* P = reinterpret_cast<const char*>(base + (size_t)p * plane_len);
+                const uint4 v = *reinterpret_cast<const uint4*>(P + o + 16u * j);
                 uf[j][p] = *reinterpret_cast<const bf8*>(&v);
             }
     };
