@@ -111,7 +111,9 @@ struct EnsArgs {
 // others) as FP32 U[source][member].  Workgroup = 64 sources x 64 members:
 // member-major rows are read along the sources and written along the members
 // through an LDS transpose, so both sides are coalesced.  Requires B == 64.
-__global__ __launch_bounds__(256) void ens_prep_u_kernel(const double* __restrict__ V1,
+// one workgroup per 64 sources (n / 64 of them, about one wave per SIMD):
+// registers go to loads in flight, not to occupancy
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 2))) void ens_prep_u_kernel(const double* __restrict__ V1,
                                                          const double* __restrict__ V2,
                                                          const double* __restrict__ c0,
                                                          const double* __restrict__ s0, double Vpl, long long n,
@@ -126,17 +128,33 @@ __global__ __launch_bounds__(256) void ens_prep_u_kernel(const double* __restric
         if (tid == 0) ctrl->gsteps += 1;
     }
     const long long j0 = (long long)blockIdx.x * 64;
-    for (int it = tid; it < 64 * 64; it += 256) {
-        const int m = it >> 6, jj = it & 63;
+    // thread tid covers source j0 + (tid & 63) of members (tid >> 6) + 4 k,
+    // k < 16.  The loads are unconditional (a source past n reads column
+    // n - 1 and is zeroed afterwards), so all 64 of a thread's loads are in
+    // flight at once instead of 16 rounds of flag-then-load latency.
+    {
+        const int jj = tid & 63, mb = tid >> 6;
         const long long j = j0 + jj;
-        float u1 = 0.f, u2 = 0.f;
-        if (j < n && mc[m].active) {
-            const size_t idx = (size_t)m * n + j;
-            u1 = (float)__dsub_rn(V1[idx], __dmul_rn(Vpl, c0[idx]));
-            u2 = (float)__dsub_rn(V2[idx], __dmul_rn(Vpl, s0[idx]));
+        const long long jc = j < n ? j : n - 1;
+        double v1[16], cc[16], v2[16], ss[16];
+        int on[16];
+#pragma unroll
+        for (int k = 0; k < 16; ++k) {
+            const int m = mb + 4 * k;
+            const size_t idx = (size_t)m * n + jc;
+            v1[k] = V1[idx];
+            cc[k] = c0[idx];
+            v2[k] = V2[idx];
+            ss[k] = s0[idx];
+            on[k] = mc[m].active;
         }
-        t1[m][jj] = u1;
-        t2[m][jj] = u2;
+#pragma unroll
+        for (int k = 0; k < 16; ++k) {
+            const int m = mb + 4 * k;
+            const bool use = j < n && on[k];
+            t1[m][jj] = use ? (float)__dsub_rn(v1[k], __dmul_rn(Vpl, cc[k])) : 0.f;
+            t2[m][jj] = use ? (float)__dsub_rn(v2[k], __dmul_rn(Vpl, ss[k])) : 0.f;
+        }
     }
     __syncthreads();
     for (int it = tid; it < 64 * 64; it += 256) {
@@ -569,10 +587,27 @@ __global__ __launch_bounds__(256) void ens_rates_kernel(EnsArgs A) {
     if (!A.mc[m].active) return;  // round 0 reads A.mc's first half, as here
     const long long e = t - (long long)m * n;
     double s0 = 0, s1 = 0, s2 = 0;
-    for (int c = 0; c < A.nchunk; ++c) {
-        s0 += A.partial[(((size_t)c * 3 + 0) * A.B + m) * A.npad + e];
-        s1 += A.partial[(((size_t)c * 3 + 1) * A.B + m) * A.npad + e];
-        s2 += A.partial[(((size_t)c * 3 + 2) * A.B + m) * A.npad + e];
+    // four chunks' loads in flight at a time; the sums keep chunk order
+    const size_t cs = (size_t)3 * A.B * A.npad;  // one chunk's partials
+    const float* p = A.partial + (size_t)m * A.npad + e;
+    int c = 0;
+    for (; c + 4 <= A.nchunk; c += 4) {
+        float v[4][3];
+#pragma unroll
+        for (int u = 0; u < 4; ++u)
+#pragma unroll
+            for (int q = 0; q < 3; ++q) v[u][q] = p[(size_t)(c + u) * cs + (size_t)q * A.B * A.npad];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            s0 += v[u][0];
+            s1 += v[u][1];
+            s2 += v[u][2];
+        }
+    }
+    for (; c < A.nchunk; ++c) {
+        s0 += p[(size_t)c * cs];
+        s1 += p[(size_t)c * cs + (size_t)A.B * A.npad];
+        s2 += p[(size_t)c * cs + (size_t)2 * A.B * A.npad];
     }
     double* f = A.f + t;
     f[(size_t)E_ADOTV1 * items] = s0;
