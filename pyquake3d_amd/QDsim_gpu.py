@@ -700,6 +700,41 @@ class QDsim:
                 f.write("SCALARS %s float\nLOOKUP_TABLE default\n" % label)
                 f.write("".join((fmt + " ") % v for v in vals) + "\n")
 
+    def outputtxt(self, fname):
+        """Surface grids of slip rate and traction (QDsim_gpu.py:2085-2151):
+        slipv, slipv1, slipv2 and Tt interpolated with SciPy's cubic
+        ``griddata`` from the element centroids' (x, z) onto a 500 x 300 grid
+        inset by the largest element size, written under ``out_txt/`` (in the
+        working directory) as ``X_grid.txt``, ``Y_grid.txt`` (rows of '%f ')
+        and ``<fname>slipv.txt``, ``slipv1``, ``slipv2``, ``Traction`` (rows of
+        '%.5f '; points outside the centroids' hull are nan).  The values
+        are the current device state."""
+        from scipy.interpolate import griddata  # host post-processing only
+        if self.__dict__.get("_gpu_ready"):
+            for name in ("slipv", "slipv1", "slipv2", "Tt"):
+                setattr(self, name, self.sim.get(name))
+        directory = "out_txt"
+        os.makedirs(directory, exist_ok=True)
+        xg = np.asarray(self.xg, dtype=np.float64)
+        xmin, xmax = np.min(xg[:, 0]), np.max(xg[:, 0])
+        zmin, zmax = np.min(xg[:, 2]), np.max(xg[:, 2])
+        X1 = np.linspace(xmin + self.maxsize, xmax - self.maxsize, 500)
+        Y1 = np.linspace(zmin + self.maxsize, zmax - self.maxsize, 300)
+        X_grid, Y_grid = np.meshgrid(X1, Y1)
+        pts = np.column_stack((X_grid.ravel(), Y_grid.ravel()))
+
+        def write(path, grid, fmt):
+            with open(path, "w") as f:
+                for row in grid:
+                    f.write("".join(fmt % v for v in row) + "\n")
+
+        write(os.path.join(directory, "X_grid.txt"), X_grid, "%f ")
+        write(os.path.join(directory, "Y_grid.txt"), Y_grid, "%f ")
+        for attr, tag in (("slipv", "slipv"), ("slipv1", "slipv1"), ("slipv2", "slipv2"), ("Tt", "Traction")):
+            vals = np.asarray(getattr(self, attr), dtype=np.float64)
+            grid = griddata(xg[:, [0, 2]], vals, pts, method="cubic").reshape(len(Y1), len(X1))
+            write(os.path.join(directory, fname + tag + ".txt"), grid, "%.5f ")
+
     def Calc_P_implicit(self, dt):
         """One pore-pressure update on the device (QDsim_gpu.py:628-671); the
         step drivers already run it after every accepted step."""
