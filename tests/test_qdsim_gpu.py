@@ -212,6 +212,17 @@ def test_dropin_qdsim_api(tmp_path):
     import re
     labels = re.findall(r"SCALARS (\S+) float", "\n".join(lines))  # the first shares the CELL_DATA line
     assert labels == [f[0] for f in QDsim_gpu.QDsim._VTK_FIELDS]
+    # the reference's out_txt grids, from the current device values
+    cwd = os.getcwd()
+    os.chdir(str(tmp_path))
+    try:
+        sim0.outputtxt("step30")
+    finally:
+        os.chdir(cwd)
+    assert np.array_equal(sim0.slipv1, sim0.slipv1_gpu.get()) and np.array_equal(sim0.Tt, sim0.Tt_gpu.get())
+    for tag in ("slipv", "slipv1", "slipv2", "Traction"):
+        g = np.loadtxt(tmp_path / "out_txt" / f"step30{tag}.txt")
+        assert g.shape == (300, 500) and np.isfinite(g).mean() > 0.9
     # a second construction loads the cached Green's functions
     sim1 = QDsim_gpu.QDsim(elelst, nodelst, par)
     assert np.array_equal(sim1.A1s, sim0.A1s)
