@@ -973,7 +973,9 @@ __device__ __forceinline__ double rd_lane(double v, int l) {
 
 // out[t] = raw[t] - sum over l < k with c_l != 0 of c_l B[l ldb + t], t < len,
 // the terms in order of l; c_l = C[l ldc] (k <= 64, one per lane).  Four
-// entries per lane per pass keep four loads in flight.
+// entries per lane per pass and four terms at a time keep sixteen loads in
+// flight (one term at a time, a 15k-row residual waited out one memory
+// latency per term and pass).
 __device__ __forceinline__ void dresidual(int k, const double* C, long long ldc, const double* B, long long ldb,
                           const double* raw, int len, double* out) {
     const int lane = threadIdx.x & 63;
@@ -985,7 +987,30 @@ __device__ __forceinline__ void dresidual(int k, const double* C, long long ldc,
             const int t = base + lane + 64 * c;
             v[c] = t < len ? raw[t] : 0.0;
         }
-        for (int l = 0; l < k; ++l) {
+        // four terms' loads in flight (16 per lane), then their updates in
+        // term order; a zero coefficient leaves v as it is (as skipping it would)
+        int l = 0;
+        for (; l + 4 <= k; l += 4) {
+            double a4[4], bb[4][4];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                a4[u] = rd_lane(coef, l + u);
+                const double* b = B + (long long)(l + u) * ldb;
+#pragma unroll
+                for (int c = 0; c < 4; ++c) {
+                    const int t = base + lane + 64 * c;
+                    bb[u][c] = t < len ? b[t] : 0.0;
+                }
+            }
+#pragma unroll
+            for (int u = 0; u < 4; ++u)
+#pragma unroll
+                for (int c = 0; c < 4; ++c) {
+                    const int t = base + lane + 64 * c;
+                    if (t < len && a4[u] != 0.0) v[c] -= a4[u] * bb[u][c];
+                }
+        }
+        for (; l < k; ++l) {
             const double a = rd_lane(coef, l);
             if (a == 0.0) continue;
             const double* b = B + (long long)l * ldb;
