@@ -379,7 +379,7 @@ __device__ __forceinline__ void ens_gemm3_tile(const float* __restrict__ A1s, co
     // wave-uniform column base plus a 32-bit per-lane byte offset (ld < 2^25,
     // checked on the host); the empty asm keeps the offset a 32-bit value, so
     // the loads take the scalar-base form instead of sixteen per-lane 64-bit
-    // pointers advanced every step (bit-identical, 2 % faster at 20k).
+    // pointers advanced every step (bit-identical, 4-6 % faster at 20k).
     const unsigned row4 = (unsigned)(row * 4), ld4 = (unsigned)(ld * 4);
     auto load_a = [&](float4 (&a)[2][8], const float* M0, const float* M1, long long kb) {
         const int lim = (int)(n - 1 - kb < 31 ? n - 1 - kb : 31);
