@@ -1750,6 +1750,48 @@ __global__ __launch_bounds__(256) void aca_dev_recompress_kernel(const DAca* __r
     if (lane == 0) rank[x] = r;
 }
 
+// out[l len + t] = sum over i < k with c[l k + i] != 0 of c[l k + i] Q[i len + t]
+// (the terms in order of i), l < r, t < len.  Eight output terms per pass: each
+// Q entry is read once per eight terms (once per term, the 60k build re-read
+// about 130 GB), four rows of Q in flight at a time.
+__device__ __forceinline__ void form_terms(const double* __restrict__ c, const double* __restrict__ Q, int k,
+                                           int len, int r, double* __restrict__ out) {
+    constexpr int G = 8;
+    for (int l0 = 0; l0 < r; l0 += G) {
+        for (int t = threadIdx.x; t < len; t += blockDim.x) {
+            double u[G];
+#pragma unroll
+            for (int j = 0; j < G; ++j) u[j] = 0.0;
+            int i = 0;
+            for (; i + 4 <= k; i += 4) {
+                double q[4];
+#pragma unroll
+                for (int a = 0; a < 4; ++a) q[a] = Q[(long long)(i + a) * len + t];
+#pragma unroll
+                for (int a = 0; a < 4; ++a)
+#pragma unroll
+                    for (int j = 0; j < G; ++j)
+                        if (l0 + j < r) {
+                            const double w = c[(l0 + j) * k + i + a];
+                            if (w != 0.0) u[j] += w * q[a];
+                        }
+            }
+            for (; i < k; ++i) {
+                const double q = Q[(long long)i * len + t];
+#pragma unroll
+                for (int j = 0; j < G; ++j)
+                    if (l0 + j < r) {
+                        const double w = c[(l0 + j) * k + i];
+                        if (w != 0.0) u[j] += w * q;
+                    }
+            }
+#pragma unroll
+            for (int j = 0; j < G; ++j)
+                if (l0 + j < r) out[(long long)(l0 + j) * len + t] = u[j];
+        }
+    }
+}
+
 // final factors in the fetch layout: U2 (m x r, column-major) at dU + uoff[x],
 // V2 (r x n, row-major) at dV + voff[x]; U2 = Qu Wh, V2 = Zh^T Qv^T, or with
 // ident the first k terms as they are (no recompression)
@@ -1778,24 +1820,8 @@ __global__ __launch_bounds__(256) void aca_dev_form_kernel(const DAca* __restric
     }
     const double* Wh = ws + wsoff[x];
     const double* Zh = Wh + k * k;
-    for (int l = 0; l < r; ++l) {
-        for (int t = threadIdx.x; t < m; t += blockDim.x) {
-            double u = 0;
-            for (int i = 0; i < k; ++i) {
-                const double w = Wh[l * k + i];
-                if (w != 0.0) u += w * Qu[(long long)i * m + t];
-            }
-            U2[(long long)l * m + t] = u;
-        }
-        for (int t = threadIdx.x; t < n; t += blockDim.x) {
-            double v = 0;
-            for (int i = 0; i < k; ++i) {
-                const double z = Zh[l * k + i];
-                if (z != 0.0) v += z * Qv[(long long)i * n + t];
-            }
-            V2[(long long)l * n + t] = v;
-        }
-    }
+    form_terms(Wh, Qu, k, m, r, U2);
+    form_terms(Zh, Qv, k, n, r, V2);
 }
 
 }  // namespace pq3d
