@@ -120,6 +120,26 @@ int pq3d_dense_set_params(void* sim, double V0, double Vpl, double eta, int fix_
 int pq3d_dense_set_time(void* sim, double t);
 /* out6 = time, next dt, last h, last dtnext, max slip rate, last error2 */
 int pq3d_dense_state(void* sim, double* out6, int* nrjct, int* status, long long* steps);
+/* Relerrormax1 / Relerrormax2 (src/QDsim_gpu.py:1221-1238, read by the
+ * reference driver's CPU branch, src/main_gpu.py:214): the relative-error
+ * maxima (+1e-10) of the last accepted step, or of the last pq3d_dense_dp45
+ * attempt if one came after it */
+int pq3d_dense_errors(void* sim, double* err1, double* err2);
+/* Replaces derivative_gpu(Tno, Tt1o, Tt2o, state) (src/QDsim_gpu.py:830-891):
+ * the rate-and-state RHS at the given device vectors (n each: MPa, MPa, MPa,
+ * dimensionless) with the frozen traction rates of the AdotV1 / AdotV2 /
+ * dsigmadt fields and the current slip rate; d_out4 (device, 4 x n) receives
+ * the reference's 4-tuple dstate/dt, dsigma/dt, dtau1/dt, dtau2/dt (MPa/s) */
+int pq3d_dense_derivative(void* sim, const double* d_Tno, const double* d_Tt1o, const double* d_Tt2o,
+                          const double* d_state, double* d_out4);
+/* Replaces RungeKutte_solve_Dormand_Prince_gpu(h) (src/QDsim_gpu.py:1128-1262)
+ * and, with err_mode 0, RungeKutte_solve_Dormand_Prince(h) (:1266-1397): one
+ * Dormand-Prince attempt of step h from the current state with the frozen
+ * traction rates, nothing committed.  d_out4 (device, 4 x n): the solution
+ * Tno, Tt1o, Tt2o, state; out4 (host): condition1, condition2, hnew1, hnew2
+ * (the rest of the reference's 8-tuple).  err_mode 1 drops the
+ * normal-traction error as the GPU method does (:1238), 0 keeps it (:1373). */
+int pq3d_dense_dp45(void* sim, double h, int err_mode, double* d_out4, double* out4);
 /* traction rates from the current slip rates into AdotV1/AdotV2/dsigmadt */
 int pq3d_dense_matvec(void* sim);
 int pq3d_dense_time_matvec(void* sim, int reps, float* ms_per_launch);
@@ -143,6 +163,10 @@ int pq3d_dense_step(void* sim, double dttry, double* h, double* dtnext, int* nrj
 /* nsteps steps without host synchronisation (HIP graph of graph_steps steps);
  * records (host, optional): nsteps x 4 = h, dtnext, time, max slip rate */
 int pq3d_dense_run(void* sim, long long nsteps, int graph_steps, double* records, int* status);
+/* after a failed step (status 1: the reference's 'dt is too small' exit,
+ * QDsim_gpu.py:983-985) every later launch of a run or graph leaves the state
+ * unchanged and its records rows are NaN; reset_ctrl clears the failure (status,
+ * grid barrier, reduction slots) keeping time, step size and counters */
 int pq3d_dense_reset_ctrl(void* sim);
 /* Dilatancy / pore-pressure diffusion (QDsim_gpu.py:336-349, :611-671):
  * zp K grid nodes, parr n x K initial profiles in Pa (row per element),
@@ -173,7 +197,10 @@ int pq3d_ens_set_field(void* ens, int field, const double* host_Bn);
 int pq3d_ens_get_field(void* ens, int field, double* host_Bn);
 int pq3d_ens_set_member_field(void* ens, int member, int field, const double* host_n);
 int pq3d_ens_get_member_field(void* ens, int member, int field, double* host_n);
-int pq3d_ens_set_params(void* ens, double V0, double Vpl, double eta);
+/* V0, Vpl, eta = mu / (2 Cs); fix_tn != 0 is the reference's Fix_Tn
+ * (src/QDsim_gpu.py:950-953): dsigma/dt is the normal loading alone and the
+ * GEMM leaves out Bs and Bd */
+int pq3d_ens_set_params(void* ens, double V0, double Vpl, double eta, int fix_tn);
 /* per member: d6 = time, next trial step, last accepted step, last dtnext,
  * t_end, 0; i6 = rejections since the last accepted step, status, active,
  * finished, accepted steps, global step at which it finished */
@@ -194,8 +221,9 @@ int pq3d_ens_time_product(void* ens, int reps, float* ms_per_product);
 /* GEMM arithmetic (no reference counterpart): mode 0 FP32 MFMA; mode 1 each
  * FP32 operand split exactly into three BF16 parts and the six leading
  * products accumulated in FP32 on the BF16 MFMA units (dropped terms 2^-24
- * of each product and below).  Mode 1 applies to the two-component GEMM
- * (Bs = Bd = 0); split_active says whether it is the one launched */
+ * of each product and below): a shear pass over A1s, A2s, A1d, A2d and,
+ * unless Bs = Bd = 0 or fix_tn, a normal pass over Bs, Bd in the same
+ * launch.  split_active says whether the split GEMM is the one launched */
 int pq3d_ens_set_precision(void* ens, int mode);
 int pq3d_ens_get_precision(void* ens, int* mode, int* split_active);
 /* zero-matrix structure of the GEMM's six matrices, as for the dense product */
@@ -229,7 +257,10 @@ int pq3d_gfsrc_destroy(void* src);
 int pq3d_gf_segments(void* src, const int* rcv_idx, long long nr_idx, const int* src_idx, long long ns_idx,
                      long long nseg, const long long* seg, double* out);
 /* ACA of the six matrices of nb blocks (rows row_idx[row_off[b]:row_off[b+1]],
- * columns likewise), relative tolerance eps, then (recompress_eps > 0) QR +
+ * columns likewise), relative tolerance eps (a negative eps selects the
+ * reference's absolute rule of src/Hmatrix.py:797,832-838: stop after a term
+ * whose pivot is below 1e-4 or whose step ||u|| ||v|| is below |eps|, the
+ * reference calling it with 0.01), then (recompress_eps > 0) QR +
  * SVD recompression to that tolerance; max_rank <= 0 for min(m, n);
  * ranks: nb x 6 out; *result holds the factors */
 int pq3d_aca_run(void* src, long long nb, const long long* row_off, const int* row_idx, const long long* col_off,

@@ -39,6 +39,7 @@ reference's (it draws its start rows from an unseeded RNG, :740), so factors
 match the reference only through the accuracy bound.
 """
 import ctypes
+import os
 import math
 import time
 
@@ -495,6 +496,14 @@ class BlockTree:
         self.perm = leaf_order(cluster_root) if cluster_root is not None else None
         self.aca_eps = 1e-4
         self.recompress = True
+        # stopping rule of the ACA: "relative" (||u|| ||v|| <= aca_eps ||S_k||_F
+        # and small reference residuals, then recompression to aca_eps), or
+        # "reference": the reference's absolute rule (Hmatrix.py:797,832-838;
+        # stop after a term with pivot < 1e-4 or step ||u|| ||v|| < aca_eps_abs,
+        # 0.01 as ACA_worker passes it, Pa per metre of slip) and no
+        # recompression, as ACA_worker (:1292-1344) leaves the factors
+        self.aca_stop = "relative"
+        self.aca_eps_abs = 0.01
         self.build_stats = {}
         self._src = None
         self._device = {}
@@ -547,9 +556,19 @@ class BlockTree:
         return self.calc_stressfunc_fromCD(row_cluster, col_cluster[Jstart:Jend])
 
     # ---- ACA (Hmatrix.py:661-1344)
+    def _aca_args(self, eps=None):
+        """(eps argument, recompression tolerance) for the stopping rule in
+        use: a negative eps selects the reference's absolute rule."""
+        if self.aca_stop == "reference":
+            return -abs(self.aca_eps_abs if eps is None else eps), 0.0
+        if self.aca_stop != "relative":
+            raise ValueError(f"aca_stop must be 'relative' or 'reference', not {self.aca_stop!r}")
+        e = self.aca_eps if eps is None else eps
+        return e, (e if self.recompress else 0.0)
+
     def _aca_one(self, row_cluster, col_cluster, eps, comps):
-        out, _ = aca_blocks(self.src, [(row_cluster, col_cluster)], eps=eps,
-                            recompress_eps=eps if self.recompress else 0.0)
+        e, rc = self._aca_args(eps)
+        out, _ = aca_blocks(self.src, [(row_cluster, col_cluster)], eps=e, recompress_eps=rc)
         d = {}
         for c in comps:
             U, V = out[0][c]
@@ -562,13 +581,15 @@ class BlockTree:
         """Low-rank factors of A1s, A2s, Bs over the block (keys U_ACA_*, V_ACA_*).
         Entries are evaluated on the GPU; ``calc_rows``/``calc_cols`` are
         accepted for signature compatibility.  ``eps`` is the relative
-        tolerance (default: the tree's ``aca_eps``)."""
-        return self._aca_one(row_cluster, col_cluster, self.aca_eps if eps is None else eps, (0, 1, 2))
+        tolerance (default: the tree's ``aca_eps``), or with
+        ``aca_stop = "reference"`` the reference's absolute step tolerance
+        (default ``aca_eps_abs``)."""
+        return self._aca_one(row_cluster, col_cluster, eps, (0, 1, 2))
 
     def ACA_plus_full_traceD(self, n_rows, n_cols, calc_rows, calc_cols, row_cluster, col_cluster, eps=None,
                              max_iter=None, verbose=False):
         """Low-rank factors of A1d, A2d, Bd (see ``ACA_plus_full_traceS``)."""
-        return self._aca_one(row_cluster, col_cluster, self.aca_eps if eps is None else eps, (3, 4, 5))
+        return self._aca_one(row_cluster, col_cluster, eps, (3, 4, 5))
 
     def _is_lowrank(self, block):
         return len(block.row_cluster) > self.thread_svd and len(block.col_cluster) > self.thread_svd
@@ -592,20 +613,35 @@ class BlockTree:
 
     def calc_Hmatrix_ACA(self, dir=None, save_corefunc=False):
         """Compress every leaf (Hmatrix.py:1599-1614): one batched ACA over all
-        low-rank leaves, one batched evaluation of all dense leaves."""
+        low-rank leaves, one batched evaluation of all dense leaves.  With
+        ``save_corefunc`` the leaves are written to
+        ``dir/blocks_to_process.joblib`` with ``joblib.dump``, as the reference
+        does (Hmatrix.py:1552-1553, :1612); this module never loads that file
+        (it is a pickle)."""
         if not self.blocks_to_process:
             self.collect_blocks(self.root_block)
         self._fill(self.blocks_to_process)
+        if save_corefunc:
+            self.save_blocks(dir)
         return self.blocks_to_process
+
+    def save_blocks(self, dir):
+        """``dir/blocks_to_process.joblib``: the leaves with their factors or
+        dense sub-blocks and flags, as plain arrays and lists."""
+        from joblib import dump
+        os.makedirs(dir, exist_ok=True)
+        path = os.path.join(dir, "blocks_to_process.joblib")
+        dump(self.blocks_to_process, path)
+        return path
 
     def _fill(self, blocks):
         lowrank = [b for b in blocks if self._is_lowrank(b)]
         dense = [b for b in blocks if not self._is_lowrank(b)]
         t0 = time.perf_counter()
         if lowrank:
-            out, st = aca_blocks(self.src, [(b.row_cluster, b.col_cluster) for b in lowrank], eps=self.aca_eps,
-                                 recompress_eps=self.aca_eps if self.recompress else 0.0,
-                                 device=self.aca_device)
+            e, rc = self._aca_args()
+            out, st = aca_blocks(self.src, [(b.row_cluster, b.col_cluster) for b in lowrank], eps=e,
+                                 recompress_eps=rc, device=self.aca_device)
             t1 = time.perf_counter()
             for b, per in zip(lowrank, out):
                 dS, dD = {}, {}

@@ -31,7 +31,7 @@ from math import sqrt
 
 import numpy as np
 
-from . import _lib, autotune, greenfunc, readmsh
+from . import _lib, autotune, greenfunc, outputs, readmsh
 from ._lib import check
 
 FIELDS = ("Tno", "Tt1o", "Tt2o", "state", "slipv1", "slipv2", "slipv", "slip1", "slip2", "slip", "Tt", "rake",
@@ -53,6 +53,9 @@ def _declare(L):
     _lib._sig(L, "pq3d_dense_set_params", [vp, d, d, d, i, i])
     _lib._sig(L, "pq3d_dense_set_time", [vp, d])
     _lib._sig(L, "pq3d_dense_state", [vp, dp, ip, ip, ctypes.POINTER(ll)])
+    _lib._sig(L, "pq3d_dense_errors", [vp, dp, dp])
+    _lib._sig(L, "pq3d_dense_derivative", [vp, vp, vp, vp, vp, vp])
+    _lib._sig(L, "pq3d_dense_dp45", [vp, d, i, vp, dp])
     _lib._sig(L, "pq3d_dense_matvec", [vp])
     _lib._sig(L, "pq3d_dense_time_matvec", [vp, i, ctypes.POINTER(ctypes.c_float)])
     _lib._sig(L, "pq3d_dense_time_product", [vp, i, ctypes.POINTER(ctypes.c_float)])
@@ -289,7 +292,36 @@ class DenseSim:
                 "nrjct": nr.value, "status": stt.value, "steps": steps.value}
 
     def reset_ctrl(self):
+        """Clear a failure (status, grid barrier, reduction slots); time, step
+        size and counters are kept."""
         check(_lib.lib().pq3d_dense_reset_ctrl(self.h), "reset_ctrl")
+
+    def errors(self):
+        """(Relerrormax1, Relerrormax2) of the last accepted step, or of the
+        last ``dp45`` attempt if one came after it."""
+        e1, e2 = ctypes.c_double(), ctypes.c_double()
+        check(_lib.lib().pq3d_dense_errors(self.h, ctypes.byref(e1), ctypes.byref(e2)), "errors")
+        return e1.value, e2.value
+
+    def field_ptr(self, name):
+        """Device address of field ``name`` (n doubles)."""
+        p = ctypes.c_void_p()
+        check(_lib.lib().pq3d_dense_field_ptr(self.h, FIELD_ID[name], ctypes.byref(p)), "field_ptr")
+        return p.value
+
+    def derivative(self, ptrs4, out_ptr):
+        """RHS at the device vectors ``ptrs4`` (Tno, Tt1o, Tt2o, state) with the
+        frozen traction rates into the 4 x n device buffer ``out_ptr``."""
+        check(_lib.lib().pq3d_dense_derivative(self.h, *[ctypes.c_void_p(p) for p in ptrs4],
+                                               ctypes.c_void_p(out_ptr)), "derivative")
+
+    def dp45(self, h, out_ptr, gpu_semantics=True):
+        """One uncommitted attempt of step h; the solution into the 4 x n device
+        buffer ``out_ptr``; returns (condition1, condition2, hnew1, hnew2)."""
+        out = np.zeros(4)
+        check(_lib.lib().pq3d_dense_dp45(self.h, float(h), int(bool(gpu_semantics)), ctypes.c_void_p(out_ptr),
+                                         _lib.ptr(out)), "dp45")
+        return tuple(float(x) for x in out)
 
     def close(self):
         if self.h:
@@ -316,6 +348,14 @@ class DeviceField:
     def set(self, arr):
         self._sim.set(self._name, arr)
 
+    @property
+    def device_ptr(self):
+        return self._sim.field_ptr(self._name)
+
+    def __array__(self, dtype=None, copy=None):
+        a = self.get()
+        return a if dtype is None else a.astype(dtype)
+
     def __len__(self):
         return self._sim.n
 
@@ -324,12 +364,68 @@ class DeviceField:
         return (self._sim.n,)
 
 
+class DeviceArray:
+    """n doubles in device memory (what the reference's CuPy arrays are to its
+    callers: results of ``derivative_gpu`` and of the Dormand-Prince attempt);
+    ``.get()`` copies them to the host.  Views of one buffer keep it alive."""
+
+    def __init__(self, n, buf=None, offset=0):
+        self.n = int(n)
+        self._buf = buf if buf is not None else _lib.DeviceBuffer(8 * max(self.n, 1))
+        self._off = int(offset)
+
+    @classmethod
+    def block(cls, n, k):
+        """k arrays of n doubles, back to back in one allocation."""
+        buf = _lib.DeviceBuffer(8 * max(int(n), 1) * k)
+        return [cls(n, buf, 8 * int(n) * j) for j in range(k)]
+
+    @classmethod
+    def from_host(cls, arr):
+        a = np.ascontiguousarray(arr, dtype=np.float64).ravel()
+        d = cls(len(a))
+        d._buf.upload(a, d._off)
+        return d
+
+    @property
+    def device_ptr(self):
+        return self._buf.ptr + self._off
+
+    def get(self):
+        return self._buf.download(np.empty(self.n), self._off)
+
+    def __array__(self, dtype=None, copy=None):
+        a = self.get()
+        return a if dtype is None else a.astype(dtype)
+
+    def __len__(self):
+        return self.n
+
+    @property
+    def shape(self):
+        return (self.n,)
+
+
+def _device_operand(x, n):
+    """(device address, keep-alive) of a state vector given as a device field,
+    a device array or host values."""
+    if isinstance(x, (DeviceField, DeviceArray)):
+        if len(x) != n:
+            raise ValueError(f"vector of length {len(x)}, expected {n}")
+        return x.device_ptr, x
+    a = np.ascontiguousarray(np.broadcast_to(np.asarray(x, dtype=np.float64), (n,)))
+    d = DeviceArray.from_host(a)
+    return d.device_ptr, d
+
+
 _GPU_ATTRS = {"Tno_gpu": "Tno", "Tt1o_gpu": "Tt1o", "Tt2o_gpu": "Tt2o", "state_gpu": "state",
               "slipv1_gpu": "slipv1", "slipv2_gpu": "slipv2", "slipv_gpu": "slipv", "slip1_gpu": "slip1",
               "slip2_gpu": "slip2", "slip_gpu": "slip", "Tt_gpu": "Tt", "rake_gpu": "rake",
               "rake0_gpu": "rake0", "fric_gpu": "fric", "a_gpu": "a", "b_gpu": "b", "dc_gpu": "dc",
               "f0_gpu": "f0", "shear_loading_gpu": "shear_loading", "normal_loading_gpu": "normal_loading",
-              "P_gpu": "P", "dPdt0_gpu": "dPdt0"}
+              "P_gpu": "P", "dPdt0_gpu": "dPdt0",
+              # the frozen traction rates of the last step (set by simu_forwardGPU, QDsim_gpu.py:951-957)
+              "AdotV1": "AdotV1", "AdotV2": "AdotV2", "dsigmadt": "dsigmadt"}
 
 
 class QDsim:
@@ -657,7 +753,8 @@ class QDsim:
             print("error: dt is too small")
             sys.exit()
         st = self.sim.state()
-        self.Relerrormax2 = st["err2"]
+        self.Relerrormax1, self.Relerrormax2 = self.sim.errors()
+        self.Relerrormax1_last, self.Relerrormax2_last = self.Relerrormax1, self.Relerrormax2
         self.time = self.time + h
         self.maxslipv0 = st["maxV"]
         return h, dtnext
@@ -682,23 +779,12 @@ class QDsim:
             for name in ("Tno", "P", "Tt", "Tt1o", "Tt2o", "rake", "state", "slipv", "slipv1", "slipv2", "fric",
                          "slip", "slip1", "slip2"):
                 setattr(self, name, self.sim.get(name))
-        nodes = np.asarray(self.nodelst, dtype=np.float64)
-        tri = np.asarray(self.elelst, dtype=np.int64) - 1
-        ne = len(tri)
-        with open(fname, "w") as f:
-            f.write("# vtk DataFile Version 3.0\ntest\nASCII\nDATASET  UNSTRUCTURED_GRID\n")
-            f.write("POINTS %d float\n" % len(nodes))
-            f.write("".join("%f %f %f\n" % tuple(p) for p in nodes))
-            f.write("CELLS %d %d\n" % (ne, ne * 4))
-            f.write("".join("3 %d %d %d\n" % tuple(t) for t in tri))
-            f.write("CELL_TYPES %d\n" % ne)
-            f.write("5 " * ne + "\n")
-            f.write("CELL_DATA %d " % ne)
-            for label, attr, scale, fmt in self._VTK_FIELDS:
-                vals = (np.asarray(self.a) - np.asarray(self.b)) if attr is None else \
-                    np.asarray(getattr(self, attr), dtype=np.float64) * scale
-                f.write("SCALARS %s float\nLOOKUP_TABLE default\n" % label)
-                f.write("".join((fmt + " ") % v for v in vals) + "\n")
+        fields = []
+        for label, attr, scale, fmt in self._VTK_FIELDS:
+            vals = (np.asarray(self.a) - np.asarray(self.b)) if attr is None else \
+                np.asarray(getattr(self, attr), dtype=np.float64) * scale
+            fields.append((label, vals, fmt))
+        outputs.write_legacy_vtk(fname, self.nodelst, np.asarray(self.elelst, dtype=np.int64) - 1, fields)
 
     def outputtxt(self, fname):
         """Surface grids of slip rate and traction (QDsim_gpu.py:2085-2151):
@@ -745,6 +831,48 @@ class QDsim:
         self.dPdt0 = self.sim.get("dPdt0")
         self.Parr = self.sim.get_dilatancy()
 
+    # ---- the stepping pieces, one call at a time
+    def derivative_gpu(self, Tno, Tt1o, Tt2o, state):
+        """The rate-and-state RHS (QDsim_gpu.py:830-891) at the given state
+        vectors (device fields such as ``Tno_gpu``, ``DeviceArray`` results or
+        host arrays) with the frozen traction rates of the last step (the
+        ``AdotV1``, ``AdotV2``, ``dsigmadt`` fields).  Returns the reference's
+        4-tuple (dstate/dt, dsigma/dt, dtau1/dt, dtau2/dt) as device arrays."""
+        if not self.__dict__.get("_gpu_ready"):
+            self.initGPUvariable()
+        n = self.sim.n
+        ops = [_device_operand(x, n) for x in (Tno, Tt1o, Tt2o, state)]
+        out = DeviceArray.block(n, 4)
+        self.sim.derivative([p for p, _ in ops], out[0].device_ptr)
+        return tuple(out)
+
+    def RungeKutte_solve_Dormand_Prince_gpu(self, h):
+        """One Dormand-Prince attempt of step h from the current device state
+        with the frozen traction rates, nothing committed (QDsim_gpu.py:1128-1262):
+        returns (Tno_yhk, Tt1o_yhk, Tt2o_yhk, state_yhk, condition1, condition2,
+        hnew1, hnew2), the vectors as device arrays; sets Relerrormax1/2.  The
+        normal-traction error is dropped as in the reference's GPU method."""
+        return self._attempt(h, True)
+
+    def _attempt(self, h, gpu_semantics):
+        if not self.__dict__.get("_gpu_ready"):
+            self.initGPUvariable()
+        out = DeviceArray.block(self.sim.n, 4)
+        c1, c2, hn1, hn2 = self.sim.dp45(h, out[0].device_ptr, gpu_semantics)
+        self.Relerrormax1, self.Relerrormax2 = self.sim.errors()
+        return out[0], out[1], out[2], out[3], c1, c2, hn1, hn2
+
+    def derivative(self, Tno, Tt1o, Tt2o, state):
+        """The CPU branch's RHS (QDsim_gpu.py:772-828; the same formulas),
+        computed on the device: host arrays in, host arrays out."""
+        return tuple(x.get() for x in self.derivative_gpu(Tno, Tt1o, Tt2o, state))
+
+    def RungeKutte_solve_Dormand_Prince(self, h):
+        """The CPU branch's attempt (QDsim_gpu.py:1266-1397): as the GPU method
+        but with the normal-traction error kept; host arrays out."""
+        r = self._attempt(h, False)
+        return tuple(x.get() for x in r[:4]) + r[4:]
+
     def simu_forwardGPU(self, dttry):
         """One accepted step on the device; returns (h, dtnext)."""
         return self._step(dttry, True)
@@ -754,7 +882,7 @@ class QDsim:
         host attributes the CPU branch maintains."""
         h, dtnext = self._step(dttry, False)
         for name in ("Tno", "Tt1o", "Tt2o", "state", "slipv1", "slipv2", "slipv", "slip1", "slip2", "slip", "Tt",
-                     "rake", "fric"):
+                     "rake", "fric", "AdotV1", "AdotV2", "dsigmadt"):
             setattr(self, name, self.sim.get(name))
         if self.Ifdila:
             self.P = self.sim.get("P")

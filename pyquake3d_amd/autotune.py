@@ -76,7 +76,10 @@ def _device_key():
 
 
 def _nq(obj):
-    """Traction components the product computes (2 when Bs and Bd are skipped)."""
+    """Traction components the product computes (2 when Bs and Bd are skipped:
+    both zero, or fix_tn)."""
+    if getattr(obj, "fix_tn", False):
+        return 2
     st = getattr(obj, "structure", None)
     return 2 if (st is not None and st[1]) else 3
 

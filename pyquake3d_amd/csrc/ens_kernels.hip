@@ -103,6 +103,7 @@ struct EnsArgs {
     long long n, npad;
     int B, nchunk;
     double V0, Vpl, eta;
+    int fix_tn;                // dsigma/dt = normal loading alone (Fix_Tn, QDsim_gpu.py:950-953)
     int rates_ready;           // ens_rates_kernel has filled the frozen traction rates
 };
 
@@ -312,14 +313,15 @@ __global__ __launch_bounds__(64) void ens_gemm_kernel(const float* __restrict__ 
 // occupancy 1: the strike half (A1s, A2s with u1) and the dip half (A1d, A2d
 // with u2) of a step alternate, each half's next loads issued as soon as its
 // registers are free.  kc is a multiple of 32.
-template <int L>
-__device__ __forceinline__ void ens_gemm3_half(const float4 (&a)[2][8], const bf8 (&uf)[4][3], v4f (&acc)[3][4][4]) {
+template <int L, int NM>
+__device__ __forceinline__ void ens_gemm3_half(const float4 (&a)[NM][8], const bf8 (&uf)[4][3],
+                                               v4f (&acc)[NM][4][4]) {
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
-        // subtile i of both matrices: element t of a fragment = source 8 (lane / 16) + t
-        bf8 f[2][3];
+        // subtile i of the half's matrices: element t of a fragment = source 8 (lane / 16) + t
+        bf8 f[NM][3];
 #pragma unroll
-        for (int q = 0; q < 2; ++q) {
+        for (int q = 0; q < NM; ++q) {
             unsigned h[4], m[4], l[4];
 #pragma unroll
             for (int t = 0; t < 4; ++t) {
@@ -344,7 +346,7 @@ __device__ __forceinline__ void ens_gemm3_half(const float4 (&a)[2][8], const bf
             const int pa = (p == 0 || p == 1 || p == 3) ? 0 : (p == 2 || p == 5) ? 1 : 2;
             const int pu = (p == 0 || p == 2 || p == 4) ? 0 : (p == 1 || p == 5) ? 1 : 2;
 #pragma unroll
-            for (int q = 0; q < 2; ++q)
+            for (int q = 0; q < NM; ++q)
 #pragma unroll
                 for (int j = 0; j < L; ++j)
                     acc[q][i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(f[q][pa], uf[j][pu], acc[q][i][j], 0, 0, 0);
@@ -352,9 +354,13 @@ __device__ __forceinline__ void ens_gemm3_half(const float4 (&a)[2][8], const bf
     }
 }
 
-template <int L>
-__device__ __forceinline__ void ens_gemm3_tile(const float* __restrict__ A1s, const float* __restrict__ A2s,
-                                               const float* __restrict__ A1d, const float* __restrict__ A2d,
+// NM matrices per half: the strike half multiplies S[0..NM) by u1, the dip
+// half D[0..NM) by u2, both into output components Q0 .. Q0 + NM - 1.  The
+// shear pass is NM = 2 (A1s, A2s | A1d, A2d -> q 0, 1), the normal pass NM = 1
+// (Bs | Bd -> q 2).  ZERO_NORMAL: the shear pass also writes the normal
+// partials as zeros (Bs = Bd = 0 or fix_Tn: no normal pass runs).
+template <int L, int NM, int Q0, bool ZERO_NORMAL>
+__device__ __forceinline__ void ens_gemm3_tile(const float* const (&S)[NM], const float* const (&D)[NM],
                                                long long ld, long long n, int kc, const uint4* __restrict__ planes,
                                                long long plane_len, float* __restrict__ partial,
                                                long long npad) {
@@ -365,14 +371,14 @@ __device__ __forceinline__ void ens_gemm3_tile(const float* __restrict__ A1s, co
     if (k1 > n) k1 = n;
     const int lr = lane & 15, g = lane >> 4;
     const long long row = r0 + 4 * lr < ld ? r0 + 4 * lr : ld - 4;
-    v4f acc[3][4][4];
+    v4f acc[NM][4][4];
 #pragma unroll
-    for (int q = 0; q < 3; ++q)
+    for (int q = 0; q < NM; ++q)
 #pragma unroll
         for (int i = 0; i < 4; ++i)
 #pragma unroll
             for (int j = 0; j < 4; ++j) acc[q][i][j] = v4f{0.f, 0.f, 0.f, 0.f};
-    float4 as[2][8], ad[2][8];
+    float4 as[NM][8], ad[NM][8];
     bf8 us[4][3], ud[4][3];
     // A at (source kb + min(8 g + t, n - 1 - kb), row): sources past n read
     // column n - 1 (their u parts are zero).  The address is the step's
@@ -381,17 +387,18 @@ __device__ __forceinline__ void ens_gemm3_tile(const float* __restrict__ A1s, co
     // the loads take the scalar-base form instead of sixteen per-lane 64-bit
     // pointers advanced every step (bit-identical, 4-6 % faster at 20k).
     const unsigned row4 = (unsigned)(row * 4), ld4 = (unsigned)(ld * 4);
-    auto load_a = [&](float4 (&a)[2][8], const float* M0, const float* M1, long long kb) {
+    auto load_a = [&](float4 (&a)[NM][8], const float* const (&M)[NM], long long kb) {
         const int lim = (int)(n - 1 - kb < 31 ? n - 1 - kb : 31);
-        const char* B0 = reinterpret_cast<const char*>(M0 + (size_t)kb * ld);
-        const char* B1 = reinterpret_cast<const char*>(M1 + (size_t)kb * ld);
+        const char* Bm[NM];
+#pragma unroll
+        for (int q = 0; q < NM; ++q) Bm[q] = reinterpret_cast<const char*>(M[q] + (size_t)kb * ld);
 #pragma unroll
         for (int t = 0; t < 8; ++t) {
             const int kk = 8 * g + t < lim ? 8 * g + t : lim;
             unsigned o = (unsigned)kk * ld4 + row4;
             asm volatile("" : "+v"(o));
-            a[0][t] = *reinterpret_cast<const float4*>(B0 + o);
-            a[1][t] = *reinterpret_cast<const float4*>(B1 + o);
+#pragma unroll
+            for (int q = 0; q < NM; ++q) a[q][t] = *reinterpret_cast<const float4*>(Bm[q] + o);
         }
     };
     // the members' parts: the step's uniform plane base plus a per-lane byte
@@ -410,57 +417,99 @@ __device__ __forceinline__ void ens_gemm3_tile(const float* __restrict__ A1s, co
                 uf[j][p] = *reinterpret_cast<const bf8*>(&v);
             }
     };
-    load_a(as, A1s, A2s, k0);
+    load_a(as, S, k0);
     load_u(us, 0, k0);
-    load_a(ad, A1d, A2d, k0);
+    load_a(ad, D, k0);
     load_u(ud, 1, k0);
     for (long long kb = k0; kb < k1; kb += 32) {
         const bool more = kb + 32 < k1;
-        ens_gemm3_half<L>(as, us, acc);   // A1s u1 -> q 0, A2s u1 -> q 1
+        ens_gemm3_half<L, NM>(as, us, acc);   // strike matrices x u1
         if (more) {
-            load_a(as, A1s, A2s, kb + 32);
+            load_a(as, S, kb + 32);
             load_u(us, 0, kb + 32);
         }
-        ens_gemm3_half<L>(ad, ud, acc);   // A1d u2, A2d u2 into the same q
+        ens_gemm3_half<L, NM>(ad, ud, acc);   // dip matrices x u2 into the same components
         if (more) {
-            load_a(ad, A1d, A2d, kb + 32);
+            load_a(ad, D, kb + 32);
             load_u(ud, 1, kb + 32);
         }
     }
-    // C fragment as in ens_gemm_tile; the normal component is written as zeros
+    // C fragment as in ens_gemm_tile
 #pragma unroll
-    for (int q = 0; q < 3; ++q)
+    for (int q = 0; q < NM; ++q)
 #pragma unroll
         for (int j = 0; j < 4; ++j) {
             const int member = 4 * lr + j;
 #pragma unroll
             for (int v = 0; v < 4; ++v) {
                 const long long prow = r0 + 4 * (4 * g + v);
-                float4 o = make_float4(acc[q][0][j][v], acc[q][1][j][v], acc[q][2][j][v], acc[q][3][j][v]);
-                if (q >= 2) o = make_float4(0.f, 0.f, 0.f, 0.f);
-                *reinterpret_cast<float4*>(partial + (((size_t)blockIdx.y * 3 + q) * 64 + member) * npad + prow) = o;
+                const float4 o = make_float4(acc[q][0][j][v], acc[q][1][j][v], acc[q][2][j][v], acc[q][3][j][v]);
+                *reinterpret_cast<float4*>(partial + (((size_t)blockIdx.y * 3 + Q0 + q) * 64 + member) * npad +
+                                           prow) = o;
             }
         }
+    if (ZERO_NORMAL) {
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const int member = 4 * lr + j;
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                const long long prow = r0 + 4 * (4 * g + v);
+                *reinterpret_cast<float4*>(partial + (((size_t)blockIdx.y * 3 + 2) * 64 + member) * npad + prow) =
+                    make_float4(0.f, 0.f, 0.f, 0.f);
+            }
+        }
+    }
 }
 
-// grid: (ceil(n / 64), nchunk); block 64.  Requires B == 64, ld % 4 == 0,
-// kc % 32 == 0 and Bs = Bd = 0.  Idle member subtiles are skipped as in
+template <int L>
+__device__ __forceinline__ void ens_gemm3_pass(const float* __restrict__ A1s, const float* __restrict__ A2s,
+                                               const float* __restrict__ Bs, const float* __restrict__ A1d,
+                                               const float* __restrict__ A2d, const float* __restrict__ Bd,
+                                               long long ld, long long n, int kc, const uint4* __restrict__ planes,
+                                               long long plane_len, float* __restrict__ partial, long long npad,
+                                               int with_normal) {
+    if (!with_normal) {
+        const float* const S[2] = {A1s, A2s};
+        const float* const D[2] = {A1d, A2d};
+        ens_gemm3_tile<L, 2, 0, true>(S, D, ld, n, kc, planes, plane_len, partial, npad);
+    } else if (blockIdx.z == 0) {
+        const float* const S[2] = {A1s, A2s};
+        const float* const D[2] = {A1d, A2d};
+        ens_gemm3_tile<L, 2, 0, false>(S, D, ld, n, kc, planes, plane_len, partial, npad);
+    } else {
+        const float* const S[1] = {Bs};
+        const float* const D[1] = {Bd};
+        ens_gemm3_tile<L, 1, 2, false>(S, D, ld, n, kc, planes, plane_len, partial, npad);
+    }
+}
+
+// grid: (ceil(n / 64), nchunk, 1 + with_normal); block 64.  Requires
+// B == 64, ld % 4 == 0 and kc % 32 == 0.  with_normal = 0 (Bs = Bd = 0, or
+// fix_Tn): the shear pass alone, normal partials written as zeros;
+// with_normal = 1: grid.z = 1 is the normal pass over Bs and Bd, so every
+// matrix is still read once.  Idle member subtiles are skipped as in
 // ens_gemm_kernel.
 __global__ __launch_bounds__(64) void ens_gemm3_kernel(const float* __restrict__ A1s, const float* __restrict__ A2s,
-                                                       const float* __restrict__ A1d, const float* __restrict__ A2d,
+                                                       const float* __restrict__ Bs, const float* __restrict__ A1d,
+                                                       const float* __restrict__ A2d, const float* __restrict__ Bd,
                                                        long long ld, long long n, int kc,
                                                        const uint4* __restrict__ planes, long long plane_len,
                                                        float* __restrict__ partial, long long npad,
-                                                       const MemberCtrl* __restrict__ mc) {
+                                                       const MemberCtrl* __restrict__ mc, int with_normal) {
     const unsigned long long act = __ballot(mc[threadIdx.x & 63].active != 0);
     int L = 0;
 #pragma unroll
     for (int j = 0; j < 4; ++j)
         if (act & (0x1111111111111111ull << j)) L = j + 1;
-    if (L == 4) ens_gemm3_tile<4>(A1s, A2s, A1d, A2d, ld, n, kc, planes, plane_len, partial, npad);
-    else if (L == 3) ens_gemm3_tile<3>(A1s, A2s, A1d, A2d, ld, n, kc, planes, plane_len, partial, npad);
-    else if (L == 2) ens_gemm3_tile<2>(A1s, A2s, A1d, A2d, ld, n, kc, planes, plane_len, partial, npad);
-    else ens_gemm3_tile<1>(A1s, A2s, A1d, A2d, ld, n, kc, planes, plane_len, partial, npad);
+    if (L == 4)
+        ens_gemm3_pass<4>(A1s, A2s, Bs, A1d, A2d, Bd, ld, n, kc, planes, plane_len, partial, npad, with_normal);
+    else if (L == 3)
+        ens_gemm3_pass<3>(A1s, A2s, Bs, A1d, A2d, Bd, ld, n, kc, planes, plane_len, partial, npad, with_normal);
+    else if (L == 2)
+        ens_gemm3_pass<2>(A1s, A2s, Bs, A1d, A2d, Bd, ld, n, kc, planes, plane_len, partial, npad, with_normal);
+    else
+        ens_gemm3_pass<1>(A1s, A2s, Bs, A1d, A2d, Bd, ld, n, kc, planes, plane_len, partial, npad, with_normal);
 }
 
 // ---------------------------------------------------------------------------
@@ -612,7 +661,8 @@ __global__ __launch_bounds__(256) void ens_rates_kernel(EnsArgs A) {
     double* f = A.f + t;
     f[(size_t)E_ADOTV1 * items] = s0;
     f[(size_t)E_ADOTV2 * items] = s1;
-    f[(size_t)E_DSIGMADT * items] = s2 + f[(size_t)E_NORMAL_LOADING * items];
+    f[(size_t)E_DSIGMADT * items] =
+        A.fix_tn ? f[(size_t)E_NORMAL_LOADING * items] : s2 + f[(size_t)E_NORMAL_LOADING * items];
 }
 
 // One attempt for every item of a pending member; round 0 first sums the
@@ -644,7 +694,7 @@ __global__ __launch_bounds__(256) void ens_attempt_kernel(EnsArgs A, int round) 
             }
             F(E_ADOTV1) = s0;
             F(E_ADOTV2) = s1;
-            F(E_DSIGMADT) = s2 + F(E_NORMAL_LOADING);
+            F(E_DSIGMADT) = A.fix_tn ? F(E_NORMAL_LOADING) : s2 + F(E_NORMAL_LOADING);
         }
         EElem e;
         e.a = F(E_A); e.b = F(E_B); e.dc = F(E_DC); e.f0 = F(E_F0);
@@ -776,8 +826,9 @@ struct Ensemble {
     int graph_steps = 0;
     int zero_mask = 0;    // bit k: matrix k holds only zeros (from the last scan)
     int skip_normal = 0;  // Bs = Bd = 0: the GEMM skips the normal component
+    int fix_tn = 0;       // Fix_Tn: dsigma/dt is the normal loading, the GEMM skips Bs and Bd
     // product arithmetic: 0 FP32 MFMA; 1 three-part BF16 split on the BF16
-    // MFMA units (used when skip_normal; the three-component GEMM stays FP32)
+    // MFMA units
     int prec = 0;
     uint4* planes = nullptr;  // u's BF16 parts: 2 arrays x 3 parts x (npad / 8) x 64 x 16 bytes
 };
@@ -796,6 +847,7 @@ static EnsArgs ens_args(Ensemble* e) {
     a.V0 = e->V0;
     a.Vpl = e->Vpl;
     a.eta = e->eta;
+    a.fix_tn = e->fix_tn;
     a.rates_ready = 0;
     return a;
 }
@@ -812,18 +864,24 @@ static void ens_launch_prep(Ensemble* e, hipStream_t st) {
 // the BF16-split GEMM is in use
 // (its matrix loads use 32-bit byte offsets within 32 columns: ld < 2^25)
 static bool ens_split_active(const Ensemble* e) {
-    return e->prec == 1 && e->skip_normal && e->kc % 32 == 0 && e->ld < (1LL << 25);
+    return e->prec == 1 && e->kc % 32 == 0 && e->ld < (1LL << 25);
 }
+
+// the GEMM leaves out Bs and Bd (both zero, or fix_Tn)
+static bool ens_no_normal(const Ensemble* e) { return e->skip_normal || e->fix_tn; }
 
 // the GEMM alone, on the operands of the last ens_launch_prep
 static void ens_launch_gemm_only(Ensemble* e, hipStream_t st) {
     const float* U1 = e->U;
     const float* U2 = e->U + (size_t)e->n * e->B;
     dim3 grid((unsigned)((e->n + 63) / 64), (unsigned)e->nchunk);
-    if (ens_split_active(e))
-        hipLaunchKernelGGL(ens_gemm3_kernel, grid, dim3(64), 0, st, e->mats[0], e->mats[1], e->mats[3], e->mats[4],
-                           e->ld, e->n, e->kc, e->planes, e->npad / 8 * 64, e->partial, e->npad, e->mc);
-    else if (e->skip_normal)
+    if (ens_split_active(e)) {
+        const int with_normal = ens_no_normal(e) ? 0 : 1;
+        dim3 g3((unsigned)((e->n + 63) / 64), (unsigned)e->nchunk, (unsigned)(1 + with_normal));
+        hipLaunchKernelGGL(ens_gemm3_kernel, g3, dim3(64), 0, st, e->mats[0], e->mats[1], e->mats[2], e->mats[3],
+                           e->mats[4], e->mats[5], e->ld, e->n, e->kc, e->planes, e->npad / 8 * 64, e->partial,
+                           e->npad, e->mc, with_normal);
+    } else if (ens_no_normal(e))
         hipLaunchKernelGGL(ens_gemm_kernel<2>, grid, dim3(64), 0, st, e->mats[0], e->mats[1], e->mats[2], e->mats[3],
                            e->mats[4], e->mats[5], e->ld, e->n, e->kc, U1, U2, e->partial, e->npad, e->mc);
     else
@@ -1023,12 +1081,18 @@ int pq3d_ens_get_member_field(void* h, int m, int field, double* host_n) {
     return 0;
 }
 
-int pq3d_ens_set_params(void* h, double V0, double Vpl, double eta) {
+// V0, Vpl, eta = mu / (2 Cs); fix_tn != 0: dsigma/dt is the normal loading
+// alone and the GEMM leaves out Bs and Bd (the reference's Fix_Tn,
+// QDsim_gpu.py:950-953)
+int pq3d_ens_set_params(void* h, double V0, double Vpl, double eta, int fix_tn) {
     clear_error();
     Ensemble* e = (Ensemble*)h;
+    PQ_CHECK(e, "pq3d_ens_set_params: null handle");
+    PQ_HIP(hipStreamSynchronize(e->stream));
     e->V0 = V0;
     e->Vpl = Vpl;
     e->eta = eta;
+    e->fix_tn = fix_tn ? 1 : 0;
     ens_drop_graph(e);
     return 0;
 }
@@ -1126,8 +1190,9 @@ int pq3d_ens_set_kc(void* h, int kc) {
 
 // Product arithmetic (no reference counterpart): 0 FP32 MFMA; 1 every FP32
 // operand split into three BF16 parts, six products on the BF16 MFMA units
-// with FP32 accumulation (the two-component GEMM of Bs = Bd = 0 only; the
-// three-component one stays FP32).  Mode 1 rounds kc up to a multiple of 32.
+// with FP32 accumulation (the shear pass, plus a normal pass over Bs and Bd
+// unless they are zero or fix_Tn holds).  Mode 1 rounds kc up to a multiple
+// of 32.
 int pq3d_ens_set_precision(void* h, int mode) {
     clear_error();
     Ensemble* e = (Ensemble*)h;

@@ -22,6 +22,14 @@
 // block's largest entry over all six matrices are treated as zero, so a
 // matrix that vanishes by symmetry gets rank 0 instead of a fit to rounding
 // noise.
+//
+// A negative eps selects the reference's absolute rule instead
+// (Hmatrix.py:797,832-838, called with eps = 0.01 at :1298-1307): stop right
+// after a term whose pivot is below 1e-4 or whose step ||u|| ||v|| is below
+// |eps| (both in the matrices' units, Pa per metre of slip); no reference-
+// residual check.  The reference pivots on A1s (A1d) and stops all three
+// matrices of a slip direction by its criteria; here each matrix is its own
+// instance and stops by its own.
 #include "common.hpp"
 #include "td_field.hpp"
 
@@ -37,6 +45,8 @@
 extern "C" const char* pq3d_last_error(void);
 
 namespace pq3d {
+
+constexpr double ACA_REF_PIVOT = 1e-4;  // pivot floor of the reference's absolute stopping rule
 
 __global__ void gfsrc_setup_kernel(const double* __restrict__ P, long long n, SrcGeom<2>* __restrict__ out) {
     long long j = (long long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -397,8 +407,19 @@ static void add_term(Inst& s, double eps) {
         if (s.uc[s.jref]) s.need_rc = true;
     }
     const double step = std::sqrt(uu * vv);
-    const double floor = 1e-9 * s.scale * std::sqrt((double)m * (double)n);
-    s.pending_conv = step <= eps * std::sqrt(s.norm2) || step <= floor;
+    if (eps < 0) {
+        // the reference's absolute rule (Hmatrix.py:832-838): stop after a
+        // term whose pivot is below 1e-4 or whose step ||u|| ||v|| is below |eps|
+        if (std::fabs(delta) < ACA_REF_PIVOT || step < -eps) {
+            s.converged = true;
+            s.state = ST_DONE;
+            return;
+        }
+        s.pending_conv = false;
+    } else {
+        const double floor = 1e-9 * s.scale * std::sqrt((double)m * (double)n);
+        s.pending_conv = step <= eps * std::sqrt(s.norm2) || step <= floor;
+    }
     s.tries = 0;
     if (s.need_rr || s.need_rc) {
         start_refs(s);
@@ -1195,8 +1216,18 @@ __device__ __forceinline__ void dadd_term(DAca& s, const DView& w, double eps) {
     }
     __threadfence_block();
     const double step = sqrt(uu * vv);
-    const double floor_ = 1e-9 * s.scale * sqrt((double)m * (double)n);
-    s.pending_conv = step <= eps * sqrt(s.norm2) || step <= floor_;
+    if (eps < 0) {
+        // the reference's absolute rule, as in add_term
+        if (fabs(delta) < ACA_REF_PIVOT || step < -eps) {
+            s.converged = 1;
+            s.state = ST_DONE;
+            return;
+        }
+        s.pending_conv = 0;
+    } else {
+        const double floor_ = 1e-9 * s.scale * sqrt((double)m * (double)n);
+        s.pending_conv = step <= eps * sqrt(s.norm2) || step <= floor_;
+    }
     s.tries = 0;
     if (s.need_rr || s.need_rc) {
         dstart_refs(s, w);
@@ -1938,7 +1969,8 @@ int pq3d_gf_segments(void* h, const int* rcv_idx, long long nr_idx, const int* s
 static int aca_check(GFSource* g, long long nb, const long long* row_off, const int* row_idx,
                      const long long* col_off, const int* col_idx, double eps) {
     PQ_CHECK(g, "pq3d_aca_run: null source");
-    PQ_CHECK(eps > 0 && eps < 1, "pq3d_aca_run: eps must be in (0, 1)");
+    PQ_CHECK((eps > 0 && eps < 1) || eps < 0, "pq3d_aca_run: eps must be in (0, 1), or negative for the "
+             "reference's absolute stopping rule");
     const long long nr = row_off[nb], nc = col_off[nb];
     for (long long b = 0; b < nb; ++b)
         PQ_CHECK(row_off[b + 1] > row_off[b] && col_off[b + 1] > col_off[b], "pq3d_aca_run: empty block %lld", b);

@@ -48,6 +48,8 @@ struct Ctrl {                    // device-resident step control
     unsigned bar_count, bar_gen; // grid barrier
     unsigned long long slot[2][24][2];  // per step parity, per attempt: error maxima (bits)
     unsigned long long vmax[2];         // per step parity: max slip rate (bits)
+    unsigned long long att_max[2];      // standalone attempt (pq3d_dense_dp45): error maxima (bits)
+    double att_out[4];                  // standalone attempt: condition1, condition2, hnew1, hnew2
 };
 
 struct SimArgs {
@@ -343,6 +345,11 @@ __device__ __forceinline__ void dp45(const Elem& e, double V0, double eta, const
 __global__ __launch_bounds__(256) void rk_step_kernel(SimArgs A) {
     __shared__ unsigned long long lds[16];
     Ctrl* C = A.ctrl;
+    // A failed step (status 1) or a grid-barrier timeout (status 2) ends the
+    // run: the later launches of the same run or graph leave the state, the
+    // reduction slots and their record rows untouched (the host marks those
+    // rows NaN).  Every workgroup reads the status the previous launch left.
+    if (__hip_atomic_load(&C->status, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) return;
     const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     const bool act = i < A.n;
     const long long ii = act ? i : 0;
@@ -432,6 +439,9 @@ __global__ __launch_bounds__(256) void rk_step_kernel(SimArgs A) {
         }
         return;
     }
+    // a workgroup that timed out at a barrier left the step's error maxima
+    // incomplete: no workgroup writes the post-step state then
+    if (__hip_atomic_load(&C->status, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 2) return;
     // post-step update (QDsim_gpu.py:987-1022)
     double vnow = 0.0;
     if (act) {
@@ -480,6 +490,93 @@ __global__ __launch_bounds__(256) void rk_step_kernel(SimArgs A) {
             A.records[4 * A.rec_index + 2] = C->time;
         }
     }
+}
+
+// ---------------------------------------------------------------------------
+// The reference's stepping pieces one call at a time: derivative_gpu
+// (QDsim_gpu.py:830-891) at given state vectors, and one Dormand-Prince
+// attempt RungeKutte_solve_Dormand_Prince_gpu(h) (QDsim_gpu.py:1128-1262)
+// from the current state, both with the frozen traction rates held in the
+// AdotV1, AdotV2 and dsigmadt fields.  Same arithmetic as rk_step_kernel.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ Elem frozen_elem(double* const* F, long long i) {
+    Elem e;
+    e.a = F[F_A][i];
+    e.b = F[F_B][i];
+    e.dc = F[F_DC][i];
+    e.f0 = F[F_F0][i];
+    e.P = F[F_P][i];
+    e.dPdt = F[F_DPDT0][i];
+    e.sl = F[F_SHEAR_LOADING][i];
+    e.A1 = F[F_ADOTV1][i];
+    e.A2 = F[F_ADOTV2][i];
+    e.dsig = F[F_DSIGMADT][i];
+    e.V = F[F_SLIPV][i];
+    return e;
+}
+
+// out: 4 x n = dstate/dt, dsigma/dt, dtau1/dt, dtau2/dt (MPa/s for the
+// tractions), the reference's return order
+__global__ __launch_bounds__(256) void rhs_eval_kernel(SimArgs A, const double* __restrict__ Tno,
+                                                       const double* __restrict__ Tt1, const double* __restrict__ Tt2,
+                                                       const double* __restrict__ st, double* __restrict__ out) {
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= A.n) return;
+    const Elem e = frozen_elem(A.f, i);
+    double k[4];
+    rsf_rhs(e, A.V0, A.eta, Tno[i], Tt1[i], Tt2[i], st[i], k);
+    out[i] = k[3];
+    out[A.n + i] = k[0];
+    out[2 * A.n + i] = k[1];
+    out[3 * A.n + i] = k[2];
+}
+
+// One attempt of step size h: y5 into out (4 x n: Tno, Tt1o, Tt2o, state) and
+// the two relative-error maxima into C->att_max (cleared by the caller).
+__global__ __launch_bounds__(256) void dp45_attempt_kernel(SimArgs A, double h, int err_mode,
+                                                           double* __restrict__ out) {
+    __shared__ unsigned long long lds[16];
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    const bool act = i < A.n;
+    const long long ii = act ? i : 0;
+    const Elem e = frozen_elem(A.f, ii);
+    double* const* F = A.f;
+    const double y[4] = {F[F_TNO][ii], F[F_TT1O][ii], F[F_TT2O][ii], F[F_STATE][ii]};
+    double y5[4], y4[4];
+    dp45(e, A.V0, A.eta, y, h, y5, y4);
+    double r1 = 0.0, r2 = 0.0;
+    if (act) {
+        r1 = fabs(fabs(y4[3] - y5[3]) / y4[3]);
+        const double rt1 = fabs(fabs(y4[1] - y5[1]) / y4[1]);
+        const double rt2 = fabs(fabs(y4[2] - y5[2]) / y4[2]);
+        r2 = (rt2 > rt1 || rt2 != rt2) ? rt2 : rt1;
+        if (err_mode == 0) {
+            const double rn = fabs(fabs(y4[0] - y5[0]) / y4[0]);
+            r2 = (rn > r2 || rn != rn) ? rn : r2;
+        }
+#pragma unroll
+        for (int q = 0; q < 4; ++q) out[(size_t)q * A.n + i] = y5[q];
+    }
+    block_max2_atomic(fbits(r1), fbits(r2), A.ctrl->att_max, lds);
+}
+
+// The attempt's error measures, conditions and proposed step sizes, with the
+// expressions of rk_step_kernel (QDsim_gpu.py:1221-1259); the error maxima
+// also become the control block's err1 / err2 (Relerrormax1 / 2).
+__global__ void dp45_attempt_finish_kernel(Ctrl* C, double h) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    const double e1 = bitsf(C->att_max[0]) + 1e-10;
+    const double e2 = bitsf(C->att_max[1]) + 1e-10;
+    C->err1 = e1;
+    C->err2 = e2;
+    C->att_out[0] = e1 / 1e-4;
+    C->att_out[1] = e2 / 1e-4;
+    C->att_out[2] = h * 0.9 * pow(1e-4 / e1, 0.2);
+    C->att_out[3] = h * 0.9 * pow(1e-4 / e2, 0.2);
+}
+
+__global__ void clear_u64_kernel(unsigned long long* p, int n) {
+    if ((int)threadIdx.x < n) p[threadIdx.x] = 0ull;
 }
 
 __global__ void rake_trig_kernel(const double* __restrict__ rake0, double* __restrict__ c0,
@@ -928,6 +1025,59 @@ int pq3d_dense_state(void* h, double* out6, int* nrjct, int* status, long long* 
     return 0;
 }
 
+// Relative-error measures of the last accepted step, or of the last
+// pq3d_dense_dp45 attempt if one came after it (Relerrormax1 / Relerrormax2,
+// QDsim_gpu.py:1221-1238: maxima + 1e-10).
+int pq3d_dense_errors(void* h, double* err1, double* err2) {
+    clear_error();
+    DenseSim* s = (DenseSim*)h;
+    PQ_CHECK(s && err1 && err2, "pq3d_dense_errors: null argument");
+    Ctrl c;
+    PQ_HIP(hipMemcpy(&c, s->ctrl, sizeof(Ctrl), hipMemcpyDeviceToHost));
+    *err1 = c.err1;
+    *err2 = c.err2;
+    return 0;
+}
+
+// derivative_gpu(Tno, Tt1o, Tt2o, state) (QDsim_gpu.py:830-891): the RHS at
+// the given device vectors (n each; MPa, MPa, MPa, dimensionless) with the
+// frozen traction rates of the AdotV1 / AdotV2 / dsigmadt fields and the
+// current slip rate; d_out (4 x n) receives dstate/dt, dsigma/dt, dtau1/dt,
+// dtau2/dt (MPa/s).
+int pq3d_dense_derivative(void* h, const double* d_Tno, const double* d_Tt1o, const double* d_Tt2o,
+                          const double* d_state, double* d_out) {
+    clear_error();
+    DenseSim* s = (DenseSim*)h;
+    PQ_CHECK(s && d_Tno && d_Tt1o && d_Tt2o && d_state && d_out, "pq3d_dense_derivative: null argument");
+    SimArgs a = make_args(s, nullptr, -1);
+    hipLaunchKernelGGL(rhs_eval_kernel, dim3((unsigned)((s->n + 255) / 256)), dim3(256), 0, 0, a, d_Tno, d_Tt1o,
+                       d_Tt2o, d_state, d_out);
+    PQ_HIP(hipGetLastError());
+    PQ_HIP(hipDeviceSynchronize());
+    return 0;
+}
+
+// RungeKutte_solve_Dormand_Prince_gpu(h) (QDsim_gpu.py:1128-1262): one
+// attempt of step size h from the current state with the frozen traction
+// rates; nothing is committed.  d_out (4 x n): the fifth-order solution
+// Tno, Tt1o, Tt2o, state.  out4: condition1, condition2, hnew1, hnew2.
+// err_mode 1 drops the normal-traction error (the GPU method), 0 keeps it
+// (the CPU method, QDsim_gpu.py:1373).  err1 / err2 of the control block
+// become this attempt's (pq3d_dense_errors).
+int pq3d_dense_dp45(void* h, double step, int err_mode, double* d_out, double* out4) {
+    clear_error();
+    DenseSim* s = (DenseSim*)h;
+    PQ_CHECK(s && d_out && out4, "pq3d_dense_dp45: null argument");
+    SimArgs a = make_args(s, nullptr, -1);
+    hipLaunchKernelGGL(clear_u64_kernel, dim3(1), dim3(64), 0, 0, s->ctrl->att_max, 2);
+    hipLaunchKernelGGL(dp45_attempt_kernel, dim3((unsigned)((s->n + 255) / 256)), dim3(256), 0, 0, a, step,
+                       err_mode, d_out);
+    hipLaunchKernelGGL(dp45_attempt_finish_kernel, dim3(1), dim3(64), 0, 0, s->ctrl, step);
+    PQ_HIP(hipGetLastError());
+    PQ_HIP(hipMemcpy(out4, s->ctrl->att_out, 4 * sizeof(double), hipMemcpyDeviceToHost));
+    return 0;
+}
+
 // Frozen traction rates from the current slip rates (matvec + reduction only):
 // fills the AdotV1, AdotV2, dsigmadt fields without stepping.
 __global__ void reduce_partials_kernel(const double* __restrict__ partial, int nchunk, long long n,
@@ -1141,6 +1291,12 @@ int pq3d_dense_run(void* h, long long nsteps, int graph_steps, double* records, 
         }
     }
     // records are indexed per launch; the graph path writes no records
+    long long steps0 = 0;
+    if (records) {
+        Ctrl c0;
+        PQ_HIP(hipMemcpy(&c0, s->ctrl, sizeof(Ctrl), hipMemcpyDeviceToHost));
+        steps0 = c0.steps;
+    }
     long long done = 0;
     hipStream_t st = 0;
     if (!records) {
@@ -1166,18 +1322,37 @@ int pq3d_dense_run(void* h, long long nsteps, int graph_steps, double* records, 
     PQ_HIP(hipMemcpy(&c, s->ctrl, sizeof(Ctrl), hipMemcpyDeviceToHost));
     if (records) {
         PQ_HIP(hipMemcpy(records, s->d_records, (size_t)nsteps * 4 * sizeof(double), hipMemcpyDeviceToHost));
-        const int par = (int)((c.steps + 1) & 1);
-        records[4 * (nsteps - 1) + 3] = host_bits_to_double(c.vmax[par]);
+        const long long acc = c.steps - steps0;  // rows written by accepted steps
+        if (c.status == 0) {
+            const int par = (int)((c.steps + 1) & 1);
+            records[4 * (nsteps - 1) + 3] = host_bits_to_double(c.vmax[par]);
+        } else {
+            // the failing launch logged the last accepted step's max slip
+            // rate; its row and the later ones were never written
+            for (long long r = acc > 0 ? acc : 0; r < nsteps; ++r)
+                for (int k = 0; k < 4; ++k) records[4 * r + k] = std::nan("");
+        }
     }
     *status = c.status;
     return 0;
 }
 
-// Reset the grid barrier and control block after a failed step.
+// Clear a failure (status, the grid barrier and the reduction slots) so that
+// stepping can resume; time, step size, counters and the last step's
+// measures are kept.
 int pq3d_dense_reset_ctrl(void* h) {
     clear_error();
     DenseSim* s = (DenseSim*)h;
-    PQ_HIP(hipMemset(s->ctrl, 0, sizeof(Ctrl)));
+    PQ_CHECK(s, "pq3d_dense_reset_ctrl: null handle");
+    Ctrl c;
+    PQ_HIP(hipMemcpy(&c, s->ctrl, sizeof(Ctrl), hipMemcpyDeviceToHost));
+    c.status = 0;
+    c.nrjct = 0;
+    c.bar_count = 0;
+    c.bar_gen = 0;
+    memset(c.slot, 0, sizeof(c.slot));
+    memset(c.att_max, 0, sizeof(c.att_max));
+    PQ_HIP(hipMemcpy(s->ctrl, &c, sizeof(Ctrl), hipMemcpyHostToDevice));
     return 0;
 }
 

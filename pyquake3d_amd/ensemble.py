@@ -32,7 +32,7 @@ def _declare(L):
     _lib._sig(L, "pq3d_ens_get_field", [vp, i, dp])
     _lib._sig(L, "pq3d_ens_set_member_field", [vp, i, i, dp])
     _lib._sig(L, "pq3d_ens_get_member_field", [vp, i, i, dp])
-    _lib._sig(L, "pq3d_ens_set_params", [vp, d, d, d])
+    _lib._sig(L, "pq3d_ens_set_params", [vp, d, d, d, i])
     _lib._sig(L, "pq3d_ens_members", [vp, dp, ctypes.POINTER(ll)])
     _lib._sig(L, "pq3d_ens_reset_member", [vp, i, d, d, i])
     _lib._sig(L, "pq3d_ens_gemm", [vp, i, ctypes.POINTER(ctypes.c_float)])
@@ -78,6 +78,7 @@ class Ensemble:
         self.exploit_structure = True
         self.mat_gen = 0
         self._structure_gen = None
+        self.fix_tn = False
         if precision != "fp32":
             self.set_precision(precision)
 
@@ -125,8 +126,12 @@ class Ensemble:
               "ens get member " + name)
         return out
 
-    def set_params(self, V0, Vpl, eta):
-        check(_lib.lib().pq3d_ens_set_params(self.h, V0, Vpl, eta), "ens params")
+    def set_params(self, V0, Vpl, eta, fix_tn=False):
+        """V0, Vpl, eta = mu / (2 Cs); ``fix_tn``: the reference's Fix_Tn
+        (QDsim_gpu.py:950-953), dsigma/dt is the normal loading alone and the
+        GEMM leaves out Bs and Bd."""
+        check(_lib.lib().pq3d_ens_set_params(self.h, V0, Vpl, eta, int(bool(fix_tn))), "ens params")
+        self.fix_tn = bool(fix_tn)
 
     # ---- member control
     def members(self):
@@ -172,9 +177,9 @@ class Ensemble:
     def set_precision(self, mode):
         """GEMM arithmetic: "fp32" (FP32 MFMA) or "bf16x3" (each FP32 operand
         split exactly into three BF16 parts, the six leading products on the
-        BF16 MFMA units with FP32 accumulation; used for the two-component
-        GEMM of Bs = Bd = 0, the three-component one stays FP32).  "bf16x3"
-        rounds kc up to a multiple of 32."""
+        BF16 MFMA units with FP32 accumulation: a shear pass, and a normal pass
+        over Bs and Bd unless both are zero or fix_tn is set).  "bf16x3" rounds
+        kc up to a multiple of 32."""
         check(_lib.lib().pq3d_ens_set_precision(self.h, int(self.PRECISIONS[mode])), "ens set precision")
 
     @property
@@ -230,14 +235,14 @@ def steady_state_fields(cols, V0, Vpl):
             "P": z, "dPdt0": z, "rake0": rake}
 
 
-def init_members(ens, cols_per_member, V0, Vpl, mu, Cs):
+def init_members(ens, cols_per_member, V0, Vpl, mu, Cs, fix_tn=False):
     """Steady-state initial condition for all members from init-file columns
     (B, n, 10)."""
     cols = np.asarray(cols_per_member, dtype=np.float64)
     per = [steady_state_fields(cols[m], V0, Vpl) for m in range(cols.shape[0])]
     for k in per[0]:
         ens.set(k, np.stack([p[k] for p in per]))
-    ens.set_params(V0, Vpl, mu / (2.0 * Cs))
+    ens.set_params(V0, Vpl, mu / (2.0 * Cs), fix_tn)
 
 
 def load_member(ens, m, cols, V0, Vpl, h0=1e-3, t_end=0.0):

@@ -216,6 +216,8 @@ Plate loading rate: {vpl!r}
 #output
 totaloutputsteps: {steps}
 outsteps: 50
+outputvtu: True
+outputSLIPV: False
 """
 
 

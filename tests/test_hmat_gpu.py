@@ -243,3 +243,66 @@ def test_stepping_with_hmatrix_follows_oracle(hcase):
         got, want = s.get(name), getattr(orc, name)
         assert np.abs(got - want).max() <= 1e-6 * np.abs(want).max(), name
     s.close()
+
+
+def test_blocks_cache_written_with_joblib(hcase, tmp_path):
+    """calc_Hmatrix_ACA(dir, save_corefunc=True) writes the leaves as the
+    reference does (Hmatrix.py:1552-1553); this test loads its own file back
+    and compares every leaf."""
+    import joblib
+    nodes, tri, tree = hcase
+    path = tree.save_blocks(str(tmp_path / "core"))
+    assert path.endswith("blocks_to_process.joblib")
+    loaded = joblib.load(path)
+    assert len(loaded) == len(tree.blocks_to_process)
+    for a, b in zip(loaded, tree.blocks_to_process):
+        assert np.array_equal(a.row_cluster, b.row_cluster) and np.array_equal(a.col_cluster, b.col_cluster)
+        assert (a.judaca, a.judsvd, a.judproc) == (b.judaca, b.judsvd, b.judproc) and b.judproc
+        for name in NAMES:
+            if b.judaca:
+                for x, y in zip(a.factors(name), b.factors(name)):
+                    assert np.array_equal(x, y)
+            else:
+                assert np.array_equal(getattr(a, "Mf_" + name), getattr(b, "Mf_" + name))
+
+
+@pytest.mark.parametrize("device", (True, False))
+def test_aca_reference_absolute_stop(tmp_path, device):
+    """aca_stop = "reference": the reference's absolute rule (Hmatrix.py:797,
+    832-838, eps = 0.01 Pa per metre of slip, pivot floor 1e-4), raw ACA
+    terms with no recompression.  Every term but the last of each (block,
+    matrix) has step ||u|| ||v|| >= 0.01 and a residual column reaching 1e-4
+    (a pivot of at least 1e-4 lies in it); with a tolerance this small the
+    factors reproduce the blocks far below the relative rule's bar; the ranks
+    are at least the relative rule's; calc_Hmatrix_ACA(dir, True) writes the
+    joblib cache."""
+    from pyquake3d_amd import Hmatrix as H, readmsh, synth
+    nodes_km, tri = synth.rough_fault_mesh(24, 12, length_km=12.0, depth_km=5.4, seed=3)
+    nodes = nodes_km * 1e3
+    rel = H.build(nodes, tri, MU, LAM, True, mini_leaf=16, eps=EPS, aca_device=device)
+    eleVec, xg = readmsh.get_eleVec(nodes, tri, False)
+    tree = H.createHmatrix(xg, nodes, tri, eleVec, {"Shear modulus": MU, "Lame constants": LAM, "Half space": True},
+                           mini_leaf=16)
+    tree.aca_stop = "reference"
+    tree.aca_device = device
+    tree.calc_Hmatrix_ACA(str(tmp_path / "core"), save_corefunc=True)
+    assert (tmp_path / "core" / "blocks_to_process.joblib").exists()
+    low = [b for b in tree.blocks_to_process if b.judaca]
+    low_rel = [b for b in rel.blocks_to_process if b.judaca]
+    assert len(low) == len(low_rel) > 5
+    dense = tree.src.segments([(b.row_cluster, b.col_cluster) for b in low])
+    worst = 0.0
+    for b, br, A in zip(low, low_rel, dense):
+        for c, name in enumerate(NAMES):
+            U, V = b.factors(name)
+            k = U.shape[1]
+            assert k >= br.factors(name)[0].shape[1] or k == min(U.shape[0], V.shape[1])
+            for l in range(k - 1):
+                assert np.linalg.norm(U[:, l]) * np.linalg.norm(V[l]) >= 0.01
+                assert np.abs(U[:, l]).max() >= 1e-4
+            if np.linalg.norm(A[c]) > 0:
+                worst = max(worst, np.linalg.norm(A[c] - U @ V) / np.linalg.norm(A[c]))
+    print(f"reference stop ({'device' if device else 'host'}): worst relative error {worst:.2e}", flush=True)
+    assert worst <= 1e-6
+    rel.close()
+    tree.close()
