@@ -1,0 +1,301 @@
+"""GPU tests at the benchmark workloads' own sizes.
+
+* 20,000-element planar fault, 64-member ensemble (FP32 Green's functions):
+  - the GEMM, with the BF16 split and with FP32 MFMA, against the FP64 product
+    of the same FP32 matrices on 96 sampled receiver rows, per row within
+    2e-6 (BF16 split; 1.8e-6 measured) or 4e-6 (FP32 MFMA, which rounds its
+    FP32 accumulator once per 4-deep step of a ~1,500-source chunk: 2.8e-6
+    measured) of sum |a_ij u_j| (matrix rows read back exactly from the
+    device);
+  - the traction rates of four members after 20 global steps against the
+    FP64 dense product of the FP64-assembled matrices: relative L2 and
+    max-norm error <= 1e-5 per member and component (the headline's bar);
+  - the 20,000-element FP32 dense simulation steps like the FP64 one: 20
+    steps, step sizes within 1e-6.
+* 60,000-element rough fault, H-matrix with leaf 64, eta 2, ACA 1e-4:
+  - per block class (dense leaves; low-rank blocks by row count <= 256,
+    257-2048, > 2048), 40 sampled blocks checked on random <= 128 x 128
+    sub-blocks against exact FP64 entries: relative Frobenius error of the
+    FP64 factors <= 10 eps = 1e-3 (the ACA bar), of FP32-rounded factors
+    <= 2e-3, of BF16-rounded factors <= 1e-2; dense leaves in FP32 <= 1e-6;
+  - the device product (FP32 and BF16 factors) against exact-operator rows
+    (96 sampled rows x all 60,000 columns from pq3d_gf_segments): relative
+    2-norm error over the sampled rows <= 1e-5 (FP32 factors), <= 2e-5 shear
+    and <= 2e-4 normal (BF16 factors).
+* Serving over the 2k, 10k and 30k planar meshes: requests admitted into
+  ensembles of the three meshes; a request's result is bit-identical alone
+  and in the mix, and a 2k request's accepted steps match the NumPy oracle
+  (time and state within 1e-6).
+"""
+import ctypes
+
+import numpy as np
+import pytest
+
+from oracle import qdsim_oracle as Q
+
+pytestmark = pytest.mark.gpu
+
+MU = LAM = 32038120320.0
+CS = 3464.0
+V0, VPL = 1e-6, 1e-9
+B = 64
+
+
+def _rows_fp32(sim, k, rows, chunk=2048):
+    """Rows ``rows`` of device matrix k (FP32, source-major) as float64, read
+    back exactly in column chunks."""
+    from pyquake3d_amd import _lib
+    n, ld = sim.n, sim.ld
+    out = np.empty((len(rows), n))
+    buf = np.empty((chunk, ld), dtype=np.float32)
+    for j0 in range(0, n, chunk):
+        j1 = min(n, j0 + chunk)
+        part = buf[:j1 - j0]
+        _lib.check(_lib.lib().pq3d_memcpy_d2h(part.ctypes.data_as(ctypes.c_void_p),
+                                               ctypes.c_void_p(sim.mat_ptrs[k] + 4 * j0 * ld), part.nbytes), "d2h")
+        out[:, j0:j1] = part[:, rows].T
+    return out
+
+
+@pytest.fixture(scope="module")
+def mesh20k():
+    from pyquake3d_amd import _lib, greenfunc, readmsh, synth
+    from pyquake3d_amd.QDsim_gpu import DenseSim
+    nodes_km, tri = synth.planar_fault_mesh(100, 100, 40.0, 40.0)
+    nodes = nodes_km * 1e3
+    ev, xg = readmsh.get_eleVec(nodes, tri, False)
+    n = len(tri)
+    assert n == 20000
+    P = greenfunc.source_vertices(nodes, tri)
+    s32 = DenseSim(n, _lib.MAT_F32)
+    s32.assemble(*P, xg, ev, MU, LAM, True)
+    cols = np.stack([synth.member_params(xg, m) for m in range(B)])
+    cols[:, :, 7] = VPL * np.exp(0.5 * np.random.default_rng(4).standard_normal(cols.shape[:2]))
+    yield s32, P, xg, ev, cols
+    s32.close()
+
+
+@pytest.mark.parametrize("precision", ("bf16x3", "fp32"))
+def test_20k_ensemble_gemm_sampled_rows(mesh20k, precision):
+    from pyquake3d_amd.ensemble import Ensemble, init_members
+    s32, P, xg, ev, cols = mesh20k
+    n = s32.n
+    e = Ensemble(n, B, matrices_from=s32, precision=precision)
+    init_members(e, cols, V0, VPL, MU, CS)
+    assert e.structure[1] and e.precision == (precision, precision == "bf16x3")
+    rng = np.random.default_rng(12)
+    V1 = rng.lognormal(-18, 3, (B, n)) * rng.choice([-1, 1], (B, n))
+    V2 = rng.lognormal(-18, 3, (B, n)) * rng.choice([-1, 1], (B, n))
+    e.set("slipv1", V1)
+    e.set("slipv2", V2)
+    e.gemm()
+    got = e.gemm_result()
+    rows = np.sort(rng.choice(n, 96, replace=False))
+    r0 = cols[:, :, 0]
+    U1 = (V1 - VPL * np.cos(r0)).astype(np.float32).astype(np.float64)
+    U2 = (V2 - VPL * np.sin(r0)).astype(np.float32).astype(np.float64)
+    for q in range(2):                       # Bs = Bd = 0 on the planar fault (skipped, exactly zero)
+        As, Ad = _rows_fp32(s32, q, rows), _rows_fp32(s32, q + 3, rows)
+        want = U1 @ As.T + U2 @ Ad.T
+        mag = np.abs(U1) @ np.abs(As).T + np.abs(U2) @ np.abs(Ad).T
+        err = np.abs(got[q][:, rows] - want) / mag
+        print(f"20k {precision} component {q}: max error {err.max():.2e} of sum |a u| on 96 rows", flush=True)
+        assert err.max() <= (2e-6 if precision == "bf16x3" else 4e-6)
+    assert not np.any(got[2])
+    e.close()
+
+
+def test_20k_ensemble_traction_error(mesh20k):
+    """The headline's accuracy bar: members' traction rates from the FP32
+    Green's functions (BF16-split and FP32 GEMMs) against the FP64 dense
+    product of FP64-assembled matrices, relative error <= 1e-5."""
+    from pyquake3d_amd import _lib
+    from pyquake3d_amd.QDsim_gpu import DenseSim
+    from pyquake3d_amd.ensemble import Ensemble, init_members
+    s32, P, xg, ev, cols = mesh20k
+    n = s32.n
+    s64 = DenseSim(n, _lib.MAT_F64)
+    s64.assemble(*P, xg, ev, MU, LAM, True)
+    s64.set_params(V0, VPL, MU / (2 * CS), False, True)
+    for precision in ("bf16x3", "fp32"):
+        e = Ensemble(n, B, matrices_from=s32, precision=precision)
+        init_members(e, cols, V0, VPL, MU, CS)
+        e.run(20, graph_steps=4)
+        assert np.all(e.members()["status"] == 0)
+        e.gemm()
+        got = e.gemm_result()
+        st = {k: e.get(k) for k in ("slipv1", "slipv2", "rake0")}
+        worst = 0.0
+        for m in (0, 21, 42, 63):
+            for k in ("slipv1", "slipv2", "rake0"):
+                s64.set(k, st[k][m])
+            s64.matvec()
+            for q, name in enumerate(("AdotV1", "AdotV2")):
+                r = s64.get(name)
+                l2 = np.linalg.norm(got[q, m] - r) / np.linalg.norm(r)
+                mx = np.abs(got[q, m] - r).max() / np.abs(r).max()
+                worst = max(worst, l2, mx)
+                assert l2 <= 1e-5 and mx <= 1e-5, (precision, m, name, l2, mx)
+        print(f"20k {precision}: worst traction error vs FP64 {worst:.2e}", flush=True)
+        e.close()
+    s64.close()
+
+
+def test_20k_fp32_dense_steps_like_fp64(mesh20k):
+    """The 20,000-element workload's FP32-storage stepping against FP64
+    storage on the same FP64 assembly: 20 steps, step sizes within 1e-6."""
+    from pyquake3d_amd import _lib
+    from pyquake3d_amd.QDsim_gpu import DenseSim
+    from pyquake3d_amd.ensemble import steady_state_fields
+    s32, P, xg, ev, cols = mesh20k
+    from pyquake3d_amd import synth
+    c = synth.friction_params(xg, dc_sigma=0.2, seed=3, rake=0.5)
+    s64 = DenseSim(s32.n, _lib.MAT_F64)
+    s64.assemble(*P, xg, ev, MU, LAM, True)
+    f32 = DenseSim(s32.n, _lib.MAT_F32)
+    f32.assemble(*P, xg, ev, MU, LAM, True)
+    for s in (s64, f32):
+        for k, v in steady_state_fields(c, V0, VPL).items():
+            s.set(k, v)
+        s.set_params(V0, VPL, MU / (2 * CS), False, True)
+    d64 = d32 = 1e-3
+    for _ in range(20):
+        h64, d64, _ = s64.step(d64)
+        h32, d32, _ = f32.step(d32)
+        assert abs(h32 - h64) <= 1e-6 * h64
+    st64, st32 = s64.get("state"), f32.get("state")
+    assert np.abs(st32 - st64).max() <= 1e-6 * np.abs(st64).max()
+    s64.close()
+    f32.close()
+
+
+# ----------------------------------------------------------------- 60k H-matrix
+
+def _bf16(a):
+    f = np.ascontiguousarray(a, dtype=np.float32)
+    b = f.view(np.uint32)
+    r = ((b + 0x7FFF + ((b >> 16) & 1)) >> 16).astype(np.uint32) << 16
+    return r.view(np.float32).astype(np.float64)
+
+
+@pytest.fixture(scope="module")
+def tree60k():
+    from pyquake3d_amd import Hmatrix as H, synth
+    nodes_km, tri = synth.rough_fault_mesh(300, 100, 120.0, 30.0, amp_km=1.0, wavelength_km=40.0, hurst=0.8,
+                                           rough_ratio=1e-3, seed=7)
+    assert len(tri) == 60000
+    print("building the 60k H-matrix", flush=True)
+    tree = H.build(nodes_km * 1e3, tri, MU, LAM, True, mini_leaf=64, min_size=16, eta=2.0, eps=1e-4)
+    print("built:", {k: v for k, v in tree.build_stats.items() if k != "aca"}, flush=True)
+    yield tree
+    tree.close()
+
+
+def test_60k_block_classes(tree60k):
+    tree = tree60k
+    rng = np.random.default_rng(21)
+    names = ("A1s", "A2s", "Bs", "A1d", "A2d", "Bd")
+    classes = {"dense": [], "m<=256": [], "256<m<=2048": [], "m>2048": []}
+    for b in tree.blocks_to_process:
+        m = len(b.row_cluster)
+        classes["dense" if not b.judaca else ("m<=256" if m <= 256 else ("256<m<=2048" if m <= 2048
+                                                                         else "m>2048"))].append(b)
+    bars = {"f64": 1e-3, "f32": 2e-3, "bf16": 1e-2}
+    for key, blocks in classes.items():
+        assert len(blocks) >= 40, (key, len(blocks))
+        pick = rng.choice(len(blocks), 40, replace=False)
+        sel, reqs = [], []
+        for i in pick:
+            b = blocks[i]
+            ri = np.sort(rng.choice(len(b.row_cluster), min(128, len(b.row_cluster)), replace=False))
+            ci = np.sort(rng.choice(len(b.col_cluster), min(128, len(b.col_cluster)), replace=False))
+            sel.append((b, ri, ci))
+            reqs.append((b.row_cluster[ri], b.col_cluster[ci]))
+        exact = tree.src.segments(reqs)
+        worst = {"f64": 0.0, "f32": 0.0, "bf16": 0.0}
+        for (b, ri, ci), A in zip(sel, exact):
+            strongest = max(np.linalg.norm(A[c]) for c in range(6))
+            for c, name in enumerate(names):
+                den = max(np.linalg.norm(A[c]), 1e-8 * strongest)
+                if b.judaca:
+                    U, V = b.factors(name)
+                    if U.shape[1] == 0:
+                        assert np.linalg.norm(A[c]) <= 1e-8 * strongest
+                        continue
+                    for lab, f in (("f64", lambda x: x), ("f32", lambda x: x.astype(np.float32).astype(np.float64)),
+                                   ("bf16", _bf16)):
+                        e = np.linalg.norm(A[c] - f(U[ri]) @ f(V[:, ci])) / den
+                        worst[lab] = max(worst[lab], e)
+                else:
+                    M = np.asarray(getattr(b, "Mf_" + name))[np.ix_(ri, ci)].astype(np.float32)
+                    worst["f32"] = max(worst["f32"], np.linalg.norm(A[c] - M) / den)
+        print(f"60k class {key} ({len(blocks)} blocks): worst sub-block error "
+              + ", ".join(f"{k} {v:.2e}" for k, v in worst.items()), flush=True)
+        if key == "dense":
+            assert worst["f32"] <= 1e-6
+        else:
+            for lab, bar in bars.items():
+                assert worst[lab] <= bar, (key, lab, worst[lab])
+
+
+def test_60k_product_against_exact_rows(tree60k):
+    from pyquake3d_amd import _lib
+    tree = tree60k
+    n = len(tree.xg)
+    rng = np.random.default_rng(22)
+    rows = np.sort(rng.choice(n, 96, replace=False))
+    exact_rows = tree.src.segments([(rows, np.arange(n))])[0]      # 6 x 96 x n (receiver x source)
+    x1, x2 = rng.standard_normal(n), rng.standard_normal(n)
+    want = [exact_rows[q] @ x1 + exact_rows[q + 3] @ x2 for q in range(3)]
+    bars = {_lib.MAT_F32: (1e-5, 1e-5), _lib.MAT_BF16: (2e-5, 2e-4)}
+    for ft, (bar_shear, bar_normal) in bars.items():
+        y = tree.device(factor_type=ft).mvm(x1, x2)
+        for q in range(3):
+            err = np.linalg.norm(y[q][rows] - want[q]) / np.linalg.norm(want[q])
+            print(f"60k product ({'fp32' if ft == _lib.MAT_F32 else 'bf16'} factors) component {q}: "
+                  f"relative error {err:.2e} on 96 exact rows", flush=True)
+            assert err <= (bar_normal if q == 2 else bar_shear)
+
+
+# ----------------------------------------------------------------- serving
+
+def test_serving_2k_10k_30k():
+    from pyquake3d_amd import readmsh, synth
+    from pyquake3d_amd.serving import Request, Server, poisson_requests
+    meshes, xg = {}, {}
+    for name, (nx, nz, L, D) in {"2k": (50, 20, 50.0, 20.0), "10k": (100, 50, 100.0, 50.0),
+                                 "30k": (150, 100, 150.0, 100.0)}.items():
+        nodes_km, tri = synth.planar_fault_mesh(nx, nz, L, D)
+        nodes = nodes_km * 1e3
+        meshes[name] = (nodes, tri)
+        xg[name] = readmsh.get_eleVec(nodes, tri, False)[1]
+        assert len(tri) == {"2k": 2000, "10k": 10000, "30k": 30000}[name]
+    reqs = poisson_requests(xg, 100.0, 12, seed=3, years=(0.05, 0.3))
+    assert {r.mesh for r in reqs} == {"2k", "10k", "30k"}
+    probe = [r for r in reqs if r.mesh == "2k"][0]
+    solo = Request(rid=probe.rid, mesh=probe.mesh, cols=probe.cols, duration_s=probe.duration_s, arrival=0.0)
+    alone = Server({"2k": meshes["2k"]}, 200e9, steps_per_iteration=16, graph_steps=8)
+    alone.run([solo])
+    mats = [alone.meshes["2k"].sim.download_matrix(k) for k in range(6)]
+    alone.close()
+    mixed = Server(meshes, 200e9, steps_per_iteration=16, graph_steps=8)
+    m = mixed.run(reqs)
+    mixed.close()
+    print("serving 2k/10k/30k:", {k: m[k] for k in ("requests", "wall_s", "p50_time_to_result_s",
+                                                    "p99_time_to_result_s", "member_steps_per_s",
+                                                    "mean_occupancy")}, flush=True)
+    assert m["requests"] == len(reqs) and m["failed"] == 0
+    got = [r for r in mixed.done if r.rid == probe.rid][0]
+    ref = solo
+    assert got.steps == ref.steps and got.result["time"] == ref.result["time"]
+    for k in ("state", "slip", "slipv"):
+        assert np.array_equal(got.result[k], ref.result[k]), k
+    # the probe's accepted steps against the oracle
+    orc = Q.from_init_columns(mats, probe.cols, V0, VPL, MU, CS)
+    dt = 1e-3
+    for _ in range(ref.steps):
+        _, dt = orc.step(dt)
+    assert orc.time >= probe.duration_s
+    assert abs(ref.result["time"] - orc.time) <= 1e-6 * orc.time
+    assert np.abs(ref.result["state"] - orc.state).max() <= 1e-6 * np.abs(orc.state).max()
