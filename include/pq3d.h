@@ -97,6 +97,21 @@ int pq3d_gf_assemble(const double* P1, const double* P2, const double* P3, long 
                      const long long* rows, long long nr, const long long* cols, long long nc,
                      int src_per_block);
 
+/* No counterpart in the reference.  pq3d_gf_assemble for callers that cannot
+ * allocate device memory (tight serving loops, harnesses that forbid
+ * hipMalloc): the same arguments and results, with the staging (source
+ * geometry table, copied vertices, receivers and index lists) in a fixed
+ * 64 MiB device scratch that each GPU holds until the process ends.  Rejects
+ * (status -2) an input whose staging exceeds it: about 1,184 B per assembled
+ * source column + 72 B per source + 96 B per receiver + 8 B per index,
+ * i.e. a whole mesh of up to ~50k elements per call, larger ones in column
+ * chunks.  Calls that overlap on one GPU race on the scratch. */
+int pq3d_gf_assemble_noalloc(const double* P1, const double* P2, const double* P3, long long nsrc,
+                             const double* xg, const double* ev, long long nrcv, double mu, double lam,
+                             int halfspace, int out_type, void* const* d_out, long long ld,
+                             const long long* rows, long long nr, const long long* cols, long long nc,
+                             int src_per_block);
+
 /* ------------------------------------------------------------------------
  * 4. Dense quasi-dynamic simulation (device-resident state)
  *

@@ -16,6 +16,8 @@
 #include "common.hpp"
 #include "td_field.hpp"
 
+#include <mutex>
+
 extern "C" const char* pq3d_last_error(void);
 
 namespace pq3d {
@@ -181,6 +183,122 @@ static void fail_loudly(const char* where, double* a, size_t na, double* b, size
     for (size_t i = 0; i < nb; ++i) b[i] = NAN;
 }
 
+// ---------------------------------------------------------------------------
+// pq3d_gf_assemble and its variant that allocates nothing share the staging
+// layout and the launches; only where the staging memory comes from differs.
+// ---------------------------------------------------------------------------
+struct GfAssembleArgs {
+    const double *P1, *P2, *P3;
+    long long nsrc;
+    const double *xg, *ev;
+    long long nrcv;
+    double mu, lam;
+    int halfspace, out_type;
+    void* const* d_out;
+    long long ld;
+    const long long* rows;
+    long long nr;
+    const long long* cols;
+    long long nc;
+    int src_per_block;
+};
+
+static int gf_assemble_check(GfAssembleArgs& a) {
+    PQ_CHECK(a.out_type == MAT_F64 || a.out_type == MAT_F32, "pq3d_gf_assemble: out_type must be f64 or f32");
+    if (!a.rows) a.nr = a.nrcv;
+    if (!a.cols) a.nc = a.nsrc;
+    PQ_CHECK(a.nr >= 0 && a.nc >= 0 && a.ld >= a.nr, "pq3d_gf_assemble: bad sizes (nr=%lld nc=%lld ld=%lld)",
+             a.nr, a.nc, a.ld);
+    for (long long i = 0; a.rows && i < a.nr; ++i)
+        PQ_CHECK(a.rows[i] >= 0 && a.rows[i] < a.nrcv, "pq3d_gf_assemble: row index %lld out of range", a.rows[i]);
+    for (long long j = 0; a.cols && j < a.nc; ++j)
+        PQ_CHECK(a.cols[j] >= 0 && a.cols[j] < a.nsrc, "pq3d_gf_assemble: column index %lld out of range",
+                 a.cols[j]);
+    if (a.src_per_block <= 0) a.src_per_block = 16;
+    return 0;
+}
+
+// staging: source-geometry table (nc entries), vertices (nsrc x 9), receiver
+// positions and frames (nrcv x 12), index lists
+static size_t gf_geo_bytes(const GfAssembleArgs& a) {
+    return ((size_t)a.nc * sizeof(SrcGeom<2>) + 255) / 256 * 256;
+}
+static size_t gf_staging_bytes(const GfAssembleArgs& a) {
+    return gf_geo_bytes(a) + (size_t)a.nsrc * 9 * sizeof(double) + (size_t)a.nrcv * 12 * sizeof(double) +
+           (size_t)((a.rows ? a.nr : 0) + (a.cols ? a.nc : 0)) * sizeof(long long) + 64;
+}
+
+static int gf_assemble_staged(const GfAssembleArgs& a, char* base) {
+    const long long nsrc = a.nsrc, nrcv = a.nrcv, nr = a.nr, nc = a.nc;
+    SrcGeom<2>* d_geo = (SrcGeom<2>*)base;
+    double* d_P = (double*)(base + gf_geo_bytes(a));
+    double* d_xg = d_P + 9 * nsrc;
+    double* d_ev = d_xg + 3 * nrcv;
+    long long* d_rows = (long long*)(d_ev + 9 * nrcv);
+    long long* d_cols = d_rows + (a.rows ? nr : 0);
+    hipError_t e = hipMemcpy(d_P, a.P1, nsrc * 3 * sizeof(double), hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(d_P + 3 * nsrc, a.P2, nsrc * 3 * sizeof(double), hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(d_P + 6 * nsrc, a.P3, nsrc * 3 * sizeof(double), hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(d_xg, a.xg, nrcv * 3 * sizeof(double), hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(d_ev, a.ev, nrcv * 9 * sizeof(double), hipMemcpyHostToDevice);
+    if (e == hipSuccess && a.rows) e = hipMemcpy(d_rows, a.rows, nr * sizeof(long long), hipMemcpyHostToDevice);
+    if (e == hipSuccess && a.cols) e = hipMemcpy(d_cols, a.cols, nc * sizeof(long long), hipMemcpyHostToDevice);
+    if (e == hipSuccess) {
+        Burgers2 b{{{0.0, 1.0, 0.0}, {0.0, 0.0, 1.0}}};  // strike (Ss = 1), dip (Ds = 1)
+        hipLaunchKernelGGL((src_setup_kernel<2, Burgers2>), dim3((unsigned)((nc + 255) / 256)), dim3(256), 0, 0,
+                           d_P, d_P + 3 * nsrc, d_P + 6 * nsrc, a.cols ? d_cols : nullptr, nc, b, d_geo);
+        const int spb = a.src_per_block;
+        dim3 grid((unsigned)((nr + 255) / 256), (unsigned)((nc + spb - 1) / spb));
+        const double nu = poisson(a.mu, a.lam);
+        if (a.out_type == MAT_F64) {
+            Mat6<double> o;
+            for (int k = 0; k < 6; ++k) o.m[k] = (double*)a.d_out[k];
+            hipLaunchKernelGGL(gf_assemble_kernel<double>, grid, dim3(256), 0, 0, d_geo, d_xg, d_ev,
+                               a.rows ? d_rows : nullptr, nr, nc, spb, a.mu, a.lam, nu, a.halfspace, o, a.ld);
+        } else {
+            Mat6<float> o;
+            for (int k = 0; k < 6; ++k) o.m[k] = (float*)a.d_out[k];
+            hipLaunchKernelGGL(gf_assemble_kernel<float>, grid, dim3(256), 0, 0, d_geo, d_xg, d_ev,
+                               a.rows ? d_rows : nullptr, nr, nc, spb, a.mu, a.lam, nu, a.halfspace, o, a.ld);
+        }
+        e = hipGetLastError();
+        if (e == hipSuccess) e = hipDeviceSynchronize();
+    }
+    if (e != hipSuccess) {
+        set_error("pq3d_gf_assemble: %s", hipGetErrorString(e));
+        return -1;
+    }
+    return 0;
+}
+
+// The fixed device scratch of pq3d_gf_assemble_noalloc.  Its name is a host
+// stand-in, not a device address: host code uses only the per-device address
+// from hipGetSymbolAddress (each GPU holds its own copy of the array).
+constexpr size_t GF_NOALLOC_SCRATCH_BYTES = (size_t)64 << 20;
+__device__ __align__(256) unsigned char scratch[GF_NOALLOC_SCRATCH_BYTES];
+
+constexpr int NOALLOC_MAX_DEVICES = 64;
+static std::mutex g_noalloc_mu;
+static void* g_noalloc_ptr[NOALLOC_MAX_DEVICES] = {nullptr};
+
+// the current device's address of `scratch`, looked up on the first call on
+// that device
+static int noalloc_scratch(void** out) {
+    int dev = 0;
+    PQ_HIP(hipGetDevice(&dev));
+    PQ_CHECK(dev >= 0 && dev < NOALLOC_MAX_DEVICES, "noalloc scratch: device %d beyond %d", dev,
+             NOALLOC_MAX_DEVICES);
+    std::lock_guard<std::mutex> lock(g_noalloc_mu);
+    if (!g_noalloc_ptr[dev]) {
+        void* p = nullptr;
+        PQ_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        PQ_CHECK(p != nullptr, "noalloc scratch: no device address on device %d", dev);
+        g_noalloc_ptr[dev] = p;
+    }
+    *out = g_noalloc_ptr[dev];
+    return 0;
+}
+
 }  // namespace pq3d
 
 using namespace pq3d;
@@ -291,63 +409,39 @@ int pq3d_gf_assemble(const double* P1, const double* P2, const double* P3, long 
                      void* const* d_out, long long ld, const long long* rows, long long nr, const long long* cols,
                      long long nc, int src_per_block) {
     clear_error();
-    PQ_CHECK(out_type == MAT_F64 || out_type == MAT_F32, "pq3d_gf_assemble: out_type must be f64 or f32");
-    if (!rows) nr = nrcv;
-    if (!cols) nc = nsrc;
-    PQ_CHECK(nr >= 0 && nc >= 0 && ld >= nr, "pq3d_gf_assemble: bad sizes (nr=%lld nc=%lld ld=%lld)", nr, nc,
-             ld);
-    if (nr == 0 || nc == 0) return 0;
-    for (long long i = 0; rows && i < nr; ++i)
-        PQ_CHECK(rows[i] >= 0 && rows[i] < nrcv, "pq3d_gf_assemble: row index %lld out of range", rows[i]);
-    for (long long j = 0; cols && j < nc; ++j)
-        PQ_CHECK(cols[j] >= 0 && cols[j] < nsrc, "pq3d_gf_assemble: column index %lld out of range", cols[j]);
-    if (src_per_block <= 0) src_per_block = 16;
-    const size_t geo_bytes = ((size_t)nc * sizeof(SrcGeom<2>) + 255) / 256 * 256;
-    const size_t pb = (size_t)nsrc * 9 * sizeof(double);
-    const size_t rb = (size_t)nrcv * 12 * sizeof(double);
-    const size_t ib = ((rows ? nr : 0) + (cols ? nc : 0)) * sizeof(long long);
+    GfAssembleArgs a{P1, P2, P3, nsrc, xg, ev, nrcv, mu, lam, halfspace, out_type, d_out, ld, rows, nr, cols, nc,
+                     src_per_block};
+    if (int rc = gf_assemble_check(a)) return rc;
+    if (a.nr == 0 || a.nc == 0) return 0;
     char* base = nullptr;
-    PQ_HIP(hipMalloc((void**)&base, geo_bytes + pb + rb + ib + 64));
-    SrcGeom<2>* d_geo = (SrcGeom<2>*)base;
-    double* d_P = (double*)(base + geo_bytes);
-    double* d_xg = d_P + 9 * nsrc;
-    double* d_ev = d_xg + 3 * nrcv;
-    long long* d_rows = (long long*)(d_ev + 9 * nrcv);
-    long long* d_cols = d_rows + (rows ? nr : 0);
-    int rc = 0;
-    hipError_t e = hipMemcpy(d_P, P1, nsrc * 3 * sizeof(double), hipMemcpyHostToDevice);
-    if (e == hipSuccess) e = hipMemcpy(d_P + 3 * nsrc, P2, nsrc * 3 * sizeof(double), hipMemcpyHostToDevice);
-    if (e == hipSuccess) e = hipMemcpy(d_P + 6 * nsrc, P3, nsrc * 3 * sizeof(double), hipMemcpyHostToDevice);
-    if (e == hipSuccess) e = hipMemcpy(d_xg, xg, nrcv * 3 * sizeof(double), hipMemcpyHostToDevice);
-    if (e == hipSuccess) e = hipMemcpy(d_ev, ev, nrcv * 9 * sizeof(double), hipMemcpyHostToDevice);
-    if (e == hipSuccess && rows) e = hipMemcpy(d_rows, rows, nr * sizeof(long long), hipMemcpyHostToDevice);
-    if (e == hipSuccess && cols) e = hipMemcpy(d_cols, cols, nc * sizeof(long long), hipMemcpyHostToDevice);
-    if (e == hipSuccess) {
-        Burgers2 b{{{0.0, 1.0, 0.0}, {0.0, 0.0, 1.0}}};  // strike (Ss = 1), dip (Ds = 1)
-        hipLaunchKernelGGL((src_setup_kernel<2, Burgers2>), dim3((unsigned)((nc + 255) / 256)), dim3(256), 0, 0,
-                           d_P, d_P + 3 * nsrc, d_P + 6 * nsrc, cols ? d_cols : nullptr, nc, b, d_geo);
-        dim3 grid((unsigned)((nr + 255) / 256), (unsigned)((nc + src_per_block - 1) / src_per_block));
-        const double nu = poisson(mu, lam);
-        if (out_type == MAT_F64) {
-            Mat6<double> o;
-            for (int k = 0; k < 6; ++k) o.m[k] = (double*)d_out[k];
-            hipLaunchKernelGGL(gf_assemble_kernel<double>, grid, dim3(256), 0, 0, d_geo, d_xg, d_ev,
-                               rows ? d_rows : nullptr, nr, nc, src_per_block, mu, lam, nu, halfspace, o, ld);
-        } else {
-            Mat6<float> o;
-            for (int k = 0; k < 6; ++k) o.m[k] = (float*)d_out[k];
-            hipLaunchKernelGGL(gf_assemble_kernel<float>, grid, dim3(256), 0, 0, d_geo, d_xg, d_ev,
-                               rows ? d_rows : nullptr, nr, nc, src_per_block, mu, lam, nu, halfspace, o, ld);
-        }
-        e = hipGetLastError();
-        if (e == hipSuccess) e = hipDeviceSynchronize();
-    }
-    if (e != hipSuccess) {
-        set_error("pq3d_gf_assemble: %s", hipGetErrorString(e));
-        rc = -1;
-    }
+    PQ_HIP(hipMalloc((void**)&base, gf_staging_bytes(a)));
+    const int rc = gf_assemble_staged(a, base);
     (void)hipFree(base);
     return rc;
+}
+
+// The same assembly with its staging (source-geometry table, copied geometry
+// and index lists) in the library's fixed device scratch instead of a fresh
+// allocation: nothing is allocated or freed.  Rejects inputs whose staging
+// exceeds GF_NOALLOC_SCRATCH_BYTES.  Calls that overlap on one GPU race on
+// the scratch; every call overwrites what it reads.
+int pq3d_gf_assemble_noalloc(const double* P1, const double* P2, const double* P3, long long nsrc,
+                             const double* xg, const double* ev, long long nrcv, double mu, double lam,
+                             int halfspace, int out_type, void* const* d_out, long long ld, const long long* rows,
+                             long long nr, const long long* cols, long long nc, int src_per_block) {
+    clear_error();
+    GfAssembleArgs a{P1, P2, P3, nsrc, xg, ev, nrcv, mu, lam, halfspace, out_type, d_out, ld, rows, nr, cols, nc,
+                     src_per_block};
+    if (int rc = gf_assemble_check(a)) return rc;
+    if (a.nr == 0 || a.nc == 0) return 0;
+    const size_t need = gf_staging_bytes(a);
+    PQ_CHECK(need <= GF_NOALLOC_SCRATCH_BYTES,
+             "pq3d_gf_assemble_noalloc: staging needs %zu bytes, the scratch holds %zu (%lld sources x %d B "
+             "+ %lld vertices x 72 B + %lld receivers x 96 B + index lists); assemble in column chunks",
+             need, GF_NOALLOC_SCRATCH_BYTES, a.nc, (int)sizeof(SrcGeom<2>), a.nsrc, a.nrcv);
+    char* base = nullptr;
+    if (int rc = noalloc_scratch((void**)&base)) return rc;
+    return gf_assemble_staged(a, base);
 }
 
 }  // extern "C"
