@@ -162,10 +162,20 @@ def assembly_flops_per_pair():
         return None
 
 
+def host_threads():
+    """Host threads the CPU baselines use: the cores this process may run on,
+    capped by OMP_NUM_THREADS (the BLAS thread count) when it is set."""
+    n = len(os.sched_getaffinity(0))
+    try:
+        return max(1, min(n, int(os.environ.get("OMP_NUM_THREADS", n))))
+    except ValueError:
+        return n
+
+
 def cpu_baseline_step(cols, mats64, budget_s=15.0, max_steps=40):
     """NumPy FP64 restatement of the dense step on the host (the oracle)."""
     from oracle import qdsim_oracle as Q
-    threads = len(os.sched_getaffinity(0))
+    threads = host_threads()
     orc = Q.from_init_columns(mats64, cols, V0, VPL, MU, CS)
     orc.step(1e-3)  # untimed first step
     dt = 1.5e-3
@@ -218,7 +228,7 @@ def config1_leg(steps=2000, cpu_budget_s=8.0, graph_steps=8):
         while k < steps and time.perf_counter() - t0 < cpu_budget_s:
             _, dt = orc.step(dt)
             k += 1
-        threads = len(os.sched_getaffinity(0))
+        threads = host_threads()
         out["cpu_baseline"] = {"value": k / (time.perf_counter() - t0), "unit": "steps/s", "cores": threads,
                                "kind": "port",
                                "sample": f"{k} steps of the NumPy FP64 restatement of simu_forward on the same "
@@ -238,19 +248,28 @@ def _ref_column(args):
     return 0
 
 
+def _ref_warm(_):
+    from oracle import refbind
+    refbind.lib()
+    return 0
+
+
 def cpu_baseline_assembly(P, xg):
     """The reference's own TDstressFS_C (built from source into oracle/_ref)
-    over sampled source columns, one process per host core."""
+    over sampled source columns, one process per host thread.  The workers
+    are spawned (this process has opened the GPU, which a forked child would
+    inherit) and loaded before the timed map."""
     from oracle import refbind
     if not refbind.available():
         return None
     import multiprocessing as mp
-    threads = len(os.sched_getaffinity(0))
+    threads = host_threads()
     ncol = threads
-    t0 = time.perf_counter()
-    with mp.get_context("fork").Pool(threads) as pool:
+    with mp.get_context("spawn").Pool(threads) as pool:
+        pool.map(_ref_warm, range(threads))
+        t0 = time.perf_counter()
         pool.map(_ref_column, [(P[0][j], P[1][j], P[2][j], xg) for j in range(ncol)])
-    el = time.perf_counter() - t0
+        el = time.perf_counter() - t0
     return {"value": ncol * len(xg) / el, "unit": "pairs/s", "cores": threads, "kind": "reference",
             "sample": f"{ncol} source columns x {len(xg)} receivers through the reference TDstressFS_C "
                       f"(strike and dip calls), {threads} processes"}

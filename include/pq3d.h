@@ -126,6 +126,9 @@ int pq3d_dense_create(void** sim, long long n, int mat_type, int kc);
 int pq3d_dense_destroy(void* sim);
 /* six device matrices A1s A2s Bs A1d A2d Bd, source-major with leading dim *ld */
 int pq3d_dense_matrix_ptrs(void* sim, void** d_ptrs6, long long* ld);
+/* device memory held by the simulation (no reference counterpart; used for
+ * HBM budgets) */
+int pq3d_dense_device_bytes(void* sim, double* bytes);
 int pq3d_dense_set_field(void* sim, int field, const double* host_n);
 int pq3d_dense_get_field(void* sim, int field, double* host_n);
 int pq3d_dense_field_ptr(void* sim, int field, void** d_ptr);
@@ -208,6 +211,9 @@ int pq3d_dense_dilatancy_update(void* sim, double dt);
 int pq3d_ens_create(void** ens, long long n, int B, void* const* d_mats_or_null, long long ld, int kc);
 int pq3d_ens_destroy(void* ens);
 int pq3d_ens_matrix_ptrs(void* ens, void** d_ptrs6, long long* ld);
+/* device memory held by the ensemble (fields, operands, partial sums, the
+ * split GEMM's planes and panel copies; borrowed matrices not counted) */
+int pq3d_ens_device_bytes(void* ens, double* bytes);
 int pq3d_ens_set_field(void* ens, int field, const double* host_Bn);
 int pq3d_ens_get_field(void* ens, int field, double* host_Bn);
 int pq3d_ens_set_member_field(void* ens, int member, int field, const double* host_n);
@@ -238,7 +244,10 @@ int pq3d_ens_time_product(void* ens, int reps, float* ms_per_product);
  * products accumulated in FP32 on the BF16 MFMA units (dropped terms 2^-24
  * of each product and below): a shear pass over A1s, A2s, A1d, A2d and,
  * unless Bs = Bd = 0 or fix_tn, a normal pass over Bs, Bd in the same
- * launch.  split_active says whether the split GEMM is the one launched */
+ * launch.  Mode 1 reads panel copies of the matrices in its operand order
+ * (64 rows x 32 sources per 8 KB block), built here and whenever the
+ * structure is scanned or set: after changing the matrices, rescan.
+ * split_active says whether the split GEMM is the one launched */
 int pq3d_ens_set_precision(void* ens, int mode);
 int pq3d_ens_get_precision(void* ens, int* mode, int* split_active);
 /* zero-matrix structure of the GEMM's six matrices, as for the dense product */

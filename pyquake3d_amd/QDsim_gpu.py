@@ -54,6 +54,7 @@ def _declare(L):
     _lib._sig(L, "pq3d_dense_set_time", [vp, d])
     _lib._sig(L, "pq3d_dense_state", [vp, dp, ip, ip, ctypes.POINTER(ll)])
     _lib._sig(L, "pq3d_dense_errors", [vp, dp, dp])
+    _lib._sig(L, "pq3d_dense_device_bytes", [vp, dp])
     _lib._sig(L, "pq3d_dense_derivative", [vp, vp, vp, vp, vp, vp])
     _lib._sig(L, "pq3d_dense_dp45", [vp, d, i, vp, dp])
     _lib._sig(L, "pq3d_dense_matvec", [vp])
@@ -295,6 +296,12 @@ class DenseSim:
         """Clear a failure (status, grid barrier, reduction slots); time, step
         size and counters are kept."""
         check(_lib.lib().pq3d_dense_reset_ctrl(self.h), "reset_ctrl")
+
+    def device_bytes(self):
+        """Device memory this simulation holds."""
+        b = ctypes.c_double()
+        check(_lib.lib().pq3d_dense_device_bytes(self.h, ctypes.byref(b)), "device_bytes")
+        return b.value
 
     def errors(self):
         """(Relerrormax1, Relerrormax2) of the last accepted step, or of the

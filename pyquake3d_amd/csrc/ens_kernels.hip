@@ -361,7 +361,7 @@ __device__ __forceinline__ void ens_gemm3_half(const float4 (&a)[NM][8], const b
 // partials as zeros (Bs = Bd = 0 or fix_Tn: no normal pass runs).
 template <int L, int NM, int Q0, bool ZERO_NORMAL>
 __device__ __forceinline__ void ens_gemm3_tile(const float* const (&S)[NM], const float* const (&D)[NM],
-                                               long long ld, long long n, int kc, const uint4* __restrict__ planes,
+                                               long long KB, long long n, int kc, const uint4* __restrict__ planes,
                                                long long plane_len, float* __restrict__ partial,
                                                long long npad) {
     const int lane = threadIdx.x & 63;
@@ -370,7 +370,6 @@ __device__ __forceinline__ void ens_gemm3_tile(const float* const (&S)[NM], cons
     long long k1 = k0 + kc;
     if (k1 > n) k1 = n;
     const int lr = lane & 15, g = lane >> 4;
-    const long long row = r0 + 4 * lr < ld ? r0 + 4 * lr : ld - 4;
     v4f acc[NM][4][4];
 #pragma unroll
     for (int q = 0; q < NM; ++q)
@@ -380,25 +379,22 @@ __device__ __forceinline__ void ens_gemm3_tile(const float* const (&S)[NM], cons
             for (int j = 0; j < 4; ++j) acc[q][i][j] = v4f{0.f, 0.f, 0.f, 0.f};
     float4 as[NM][8], ad[NM][8];
     bf8 us[4][3], ud[4][3];
-    // A at (source kb + min(8 g + t, n - 1 - kb), row): sources past n read
-    // column n - 1 (their u parts are zero).  The address is the step's
-    // wave-uniform column base plus a 32-bit per-lane byte offset (ld < 2^25,
-    // checked on the host); the empty asm keeps the offset a 32-bit value, so
-    // the loads take the scalar-base form instead of sixteen per-lane 64-bit
-    // pointers advanced every step (bit-identical, 4-6 % faster at 20k).
-    const unsigned row4 = (unsigned)(row * 4), ld4 = (unsigned)(ld * 4);
-    auto load_a = [&](float4 (&a)[NM][8], const float* const (&M)[NM], long long kb) {
-        const int lim = (int)(n - 1 - kb < 31 ? n - 1 - kb : 31);
-        const char* Bm[NM];
+    // A from the panel copies: the 64-row x 32-source block (this row tile,
+    // source block kb / 32) is 8 KB contiguous, float4 t * 64 + lane holding
+    // rows 4 lr .. 4 lr + 3 of source kb + 8 g + t, so every load instruction
+    // reads 1 KB contiguously.  Rows and sources past n are stored as zeros.
+    const float* Pm[2][NM];
 #pragma unroll
-        for (int q = 0; q < NM; ++q) Bm[q] = reinterpret_cast<const char*>(M[q] + (size_t)kb * ld);
+    for (int q = 0; q < NM; ++q) {
+        Pm[0][q] = S[q] + (size_t)blockIdx.x * KB * 2048 + 4 * lane;
+        Pm[1][q] = D[q] + (size_t)blockIdx.x * KB * 2048 + 4 * lane;
+    }
+    auto load_a = [&](float4 (&a)[NM][8], int half, long long kb) {
 #pragma unroll
-        for (int t = 0; t < 8; ++t) {
-            const int kk = 8 * g + t < lim ? 8 * g + t : lim;
-            unsigned o = (unsigned)kk * ld4 + row4;
-            asm volatile("" : "+v"(o));
+        for (int q = 0; q < NM; ++q) {
+            const float* B = Pm[half][q] + (size_t)(kb >> 5) * 2048;
 #pragma unroll
-            for (int q = 0; q < NM; ++q) a[q][t] = *reinterpret_cast<const float4*>(Bm[q] + o);
+            for (int t = 0; t < 8; ++t) a[q][t] = *reinterpret_cast<const float4*>(B + 256 * t);
         }
     };
     // the members' parts: the step's uniform plane base plus a per-lane byte
@@ -417,20 +413,20 @@ __device__ __forceinline__ void ens_gemm3_tile(const float* const (&S)[NM], cons
                 uf[j][p] = *reinterpret_cast<const bf8*>(&v);
             }
     };
-    load_a(as, S, k0);
+    load_a(as, 0, k0);
     load_u(us, 0, k0);
-    load_a(ad, D, k0);
+    load_a(ad, 1, k0);
     load_u(ud, 1, k0);
     for (long long kb = k0; kb < k1; kb += 32) {
         const bool more = kb + 32 < k1;
         ens_gemm3_half<L, NM>(as, us, acc);   // strike matrices x u1
         if (more) {
-            load_a(as, S, kb + 32);
+            load_a(as, 0, kb + 32);
             load_u(us, 0, kb + 32);
         }
         ens_gemm3_half<L, NM>(ad, ud, acc);   // dip matrices x u2 into the same components
         if (more) {
-            load_a(ad, D, kb + 32);
+            load_a(ad, 1, kb + 32);
             load_u(ud, 1, kb + 32);
         }
     }
@@ -466,34 +462,34 @@ template <int L>
 __device__ __forceinline__ void ens_gemm3_pass(const float* __restrict__ A1s, const float* __restrict__ A2s,
                                                const float* __restrict__ Bs, const float* __restrict__ A1d,
                                                const float* __restrict__ A2d, const float* __restrict__ Bd,
-                                               long long ld, long long n, int kc, const uint4* __restrict__ planes,
+                                               long long KB, long long n, int kc, const uint4* __restrict__ planes,
                                                long long plane_len, float* __restrict__ partial, long long npad,
                                                int with_normal) {
     if (!with_normal) {
         const float* const S[2] = {A1s, A2s};
         const float* const D[2] = {A1d, A2d};
-        ens_gemm3_tile<L, 2, 0, true>(S, D, ld, n, kc, planes, plane_len, partial, npad);
+        ens_gemm3_tile<L, 2, 0, true>(S, D, KB, n, kc, planes, plane_len, partial, npad);
     } else if (blockIdx.z == 0) {
         const float* const S[2] = {A1s, A2s};
         const float* const D[2] = {A1d, A2d};
-        ens_gemm3_tile<L, 2, 0, false>(S, D, ld, n, kc, planes, plane_len, partial, npad);
+        ens_gemm3_tile<L, 2, 0, false>(S, D, KB, n, kc, planes, plane_len, partial, npad);
     } else {
         const float* const S[1] = {Bs};
         const float* const D[1] = {Bd};
-        ens_gemm3_tile<L, 1, 2, false>(S, D, ld, n, kc, planes, plane_len, partial, npad);
+        ens_gemm3_tile<L, 1, 2, false>(S, D, KB, n, kc, planes, plane_len, partial, npad);
     }
 }
 
-// grid: (ceil(n / 64), nchunk, 1 + with_normal); block 64.  Requires
-// B == 64, ld % 4 == 0 and kc % 32 == 0.  with_normal = 0 (Bs = Bd = 0, or
-// fix_Tn): the shear pass alone, normal partials written as zeros;
-// with_normal = 1: grid.z = 1 is the normal pass over Bs and Bd, so every
-// matrix is still read once.  Idle member subtiles are skipped as in
-// ens_gemm_kernel.
+// grid: (ceil(n / 64), nchunk, 1 + with_normal); block 64.  The six matrices
+// are the panel copies (KB source blocks of 32 per row tile).  Requires
+// B == 64 and kc % 32 == 0.  with_normal = 0 (Bs = Bd = 0, or fix_Tn): the
+// shear pass alone, normal partials written as zeros; with_normal = 1:
+// grid.z = 1 is the normal pass over Bs and Bd, so every matrix is still
+// read once.  Idle member subtiles are skipped as in ens_gemm_kernel.
 __global__ __launch_bounds__(64) void ens_gemm3_kernel(const float* __restrict__ A1s, const float* __restrict__ A2s,
                                                        const float* __restrict__ Bs, const float* __restrict__ A1d,
                                                        const float* __restrict__ A2d, const float* __restrict__ Bd,
-                                                       long long ld, long long n, int kc,
+                                                       long long KB, long long n, int kc,
                                                        const uint4* __restrict__ planes, long long plane_len,
                                                        float* __restrict__ partial, long long npad,
                                                        const MemberCtrl* __restrict__ mc, int with_normal) {
@@ -503,13 +499,37 @@ __global__ __launch_bounds__(64) void ens_gemm3_kernel(const float* __restrict__
     for (int j = 0; j < 4; ++j)
         if (act & (0x1111111111111111ull << j)) L = j + 1;
     if (L == 4)
-        ens_gemm3_pass<4>(A1s, A2s, Bs, A1d, A2d, Bd, ld, n, kc, planes, plane_len, partial, npad, with_normal);
+        ens_gemm3_pass<4>(A1s, A2s, Bs, A1d, A2d, Bd, KB, n, kc, planes, plane_len, partial, npad, with_normal);
     else if (L == 3)
-        ens_gemm3_pass<3>(A1s, A2s, Bs, A1d, A2d, Bd, ld, n, kc, planes, plane_len, partial, npad, with_normal);
+        ens_gemm3_pass<3>(A1s, A2s, Bs, A1d, A2d, Bd, KB, n, kc, planes, plane_len, partial, npad, with_normal);
     else if (L == 2)
-        ens_gemm3_pass<2>(A1s, A2s, Bs, A1d, A2d, Bd, ld, n, kc, planes, plane_len, partial, npad, with_normal);
+        ens_gemm3_pass<2>(A1s, A2s, Bs, A1d, A2d, Bd, KB, n, kc, planes, plane_len, partial, npad, with_normal);
     else
-        ens_gemm3_pass<1>(A1s, A2s, Bs, A1d, A2d, Bd, ld, n, kc, planes, plane_len, partial, npad, with_normal);
+        ens_gemm3_pass<1>(A1s, A2s, Bs, A1d, A2d, Bd, KB, n, kc, planes, plane_len, partial, npad, with_normal);
+}
+
+// Panel copy of one source-major FP32 matrix (leading dimension ld): for row
+// tile R (64 rows) and source block K (32 sources), 2048 floats at
+// (R * KB + K) * 2048, float4 t * 64 + lane = rows 64 R + 4 (lane % 16) + 0..3
+// of source 32 K + 8 (lane / 16) + t; rows and sources past n are zero.  One
+// thread per float4 of the panel.
+__global__ __launch_bounds__(256) void ens_panel_kernel(const float* __restrict__ M, long long ld, long long n,
+                                                        long long KB, long long total4, float4* __restrict__ P) {
+    const long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= total4) return;
+    const int w = (int)(idx & 511);
+    const long long blk = idx >> 9;
+    const long long R = blk / KB, K = blk - R * KB;
+    const int t = w >> 6, lane = w & 63;
+    const long long src = 32 * K + 8 * (lane >> 4) + t;
+    const long long row = 64 * R + 4 * (lane & 15);
+    float v[4] = {0.f, 0.f, 0.f, 0.f};
+    if (src < n) {
+#pragma unroll
+        for (int i = 0; i < 4; ++i)
+            if (row + i < n) v[i] = M[(size_t)src * ld + row + i];
+    }
+    P[idx] = make_float4(v[0], v[1], v[2], v[3]);
 }
 
 // ---------------------------------------------------------------------------
@@ -831,6 +851,12 @@ struct Ensemble {
     // MFMA units
     int prec = 0;
     uint4* planes = nullptr;  // u's BF16 parts: 2 arrays x 3 parts x (npad / 8) x 64 x 16 bytes
+    // panel copies of the matrices the split GEMM reads (ens_panel_kernel
+    // layout, npad x KB * 32 floats each), built when the structure is scanned
+    // or set and when the precision or fix_Tn changes
+    float* panel[6] = {nullptr};
+    long long KB = 0;         // 32-source blocks per row tile
+    int panels_ready = 0;     // the panels the split GEMM needs hold the current matrices
 };
 
 static EnsArgs ens_args(Ensemble* e) {
@@ -861,14 +887,38 @@ static void ens_launch_prep(Ensemble* e, hipStream_t st) {
                        e->U + (size_t)e->n * e->B, e->prec ? e->planes : nullptr, e->npad / 8 * 64);
 }
 
-// the BF16-split GEMM is in use
-// (its matrix loads use 32-bit byte offsets within 32 columns: ld < 2^25)
-static bool ens_split_active(const Ensemble* e) {
-    return e->prec == 1 && e->kc % 32 == 0 && e->ld < (1LL << 25);
-}
-
 // the GEMM leaves out Bs and Bd (both zero, or fix_Tn)
 static bool ens_no_normal(const Ensemble* e) { return e->skip_normal || e->fix_tn; }
+
+// the BF16-split GEMM is in use (it reads the panel copies)
+static bool ens_split_active(const Ensemble* e) { return e->prec == 1 && e->kc % 32 == 0 && e->panels_ready; }
+
+static size_t ens_panel_bytes(const Ensemble* e) { return (size_t)e->npad * (size_t)e->KB * 32 * sizeof(float); }
+
+// Build the panel copies the split GEMM needs (the four shear matrices, and
+// Bs, Bd unless the GEMM leaves them out) from the current matrices and free
+// the others; none at all outside precision mode 1.
+static int ens_update_panels(Ensemble* e) {
+    PQ_HIP(hipStreamSynchronize(e->stream));
+    e->panels_ready = 0;
+    const size_t bytes = ens_panel_bytes(e);
+    const long long total4 = (long long)(bytes / sizeof(float4));
+    for (int k = 0; k < 6; ++k) {
+        const bool need = e->prec == 1 && ((k != 2 && k != 5) || !ens_no_normal(e));
+        if (!need) {
+            if (e->panel[k]) (void)hipFree(e->panel[k]);
+            e->panel[k] = nullptr;
+            continue;
+        }
+        if (!e->panel[k]) PQ_HIP(hipMalloc((void**)&e->panel[k], bytes));
+        hipLaunchKernelGGL(ens_panel_kernel, dim3((unsigned)((total4 + 255) / 256)), dim3(256), 0, e->stream,
+                           e->mats[k], e->ld, e->n, e->KB, total4, (float4*)e->panel[k]);
+        PQ_HIP(hipGetLastError());
+    }
+    PQ_HIP(hipStreamSynchronize(e->stream));
+    e->panels_ready = e->prec == 1 ? 1 : 0;
+    return 0;
+}
 
 // the GEMM alone, on the operands of the last ens_launch_prep
 static void ens_launch_gemm_only(Ensemble* e, hipStream_t st) {
@@ -878,9 +928,9 @@ static void ens_launch_gemm_only(Ensemble* e, hipStream_t st) {
     if (ens_split_active(e)) {
         const int with_normal = ens_no_normal(e) ? 0 : 1;
         dim3 g3((unsigned)((e->n + 63) / 64), (unsigned)e->nchunk, (unsigned)(1 + with_normal));
-        hipLaunchKernelGGL(ens_gemm3_kernel, g3, dim3(64), 0, st, e->mats[0], e->mats[1], e->mats[2], e->mats[3],
-                           e->mats[4], e->mats[5], e->ld, e->n, e->kc, e->planes, e->npad / 8 * 64, e->partial,
-                           e->npad, e->mc, with_normal);
+        hipLaunchKernelGGL(ens_gemm3_kernel, g3, dim3(64), 0, st, e->panel[0], e->panel[1], e->panel[2],
+                           e->panel[3], e->panel[4], e->panel[5], e->KB, e->n, e->kc, e->planes, e->npad / 8 * 64,
+                           e->partial, e->npad, e->mc, with_normal);
     } else if (ens_no_normal(e))
         hipLaunchKernelGGL(ens_gemm_kernel<2>, grid, dim3(64), 0, st, e->mats[0], e->mats[1], e->mats[2], e->mats[3],
                            e->mats[4], e->mats[5], e->ld, e->n, e->kc, U1, U2, e->partial, e->npad, e->mc);
@@ -958,6 +1008,7 @@ int pq3d_ens_create(void** out, long long n, int B, void* const* d_mats, long lo
     }
     e->nchunk = (int)((n + e->kc - 1) / e->kc);
     e->npad = (n + 63) / 64 * 64;
+    e->KB = (n + 31) / 32;
     if (d_mats) {
         PQ_CHECK(ld >= n && ld % 4 == 0, "pq3d_ens_create: ld must be >= n and a multiple of 4");
         e->ld = ld;
@@ -1004,6 +1055,7 @@ int pq3d_ens_destroy(void* h) {
     (void)hipFree(e->f);
     (void)hipFree(e->U);
     (void)hipFree(e->planes);
+    for (int k = 0; k < 6; ++k) (void)hipFree(e->panel[k]);
     (void)hipFree(e->partial);
     (void)hipFree(e->mc);
     (void)hipFree(e->slot);
@@ -1092,8 +1144,11 @@ int pq3d_ens_set_params(void* h, double V0, double Vpl, double eta, int fix_tn) 
     e->V0 = V0;
     e->Vpl = Vpl;
     e->eta = eta;
-    e->fix_tn = fix_tn ? 1 : 0;
+    const int ft = fix_tn ? 1 : 0;
+    const bool retile = ft != e->fix_tn && e->panels_ready;
+    e->fix_tn = ft;
     ens_drop_graph(e);
+    if (retile) return ens_update_panels(e);   // the normal pass's panels come or go
     return 0;
 }
 
@@ -1206,8 +1261,9 @@ int pq3d_ens_set_precision(void* h, int mode) {
         PQ_HIP(hipMemset(e->planes, 0, bytes));  // sources past n stay zero
     }
     e->prec = mode;
-    if (mode == 1 && e->kc % 32) return pq3d_ens_set_kc(h, e->kc);
-    return 0;
+    if (mode == 1 && e->kc % 32)
+        if (int rc = pq3d_ens_set_kc(h, e->kc)) return rc;
+    return ens_update_panels(e);
 }
 
 // mode as set, and whether the BF16-split GEMM is the one launched
@@ -1220,11 +1276,14 @@ int pq3d_ens_get_precision(void* h, int* mode, int* split_active) {
     return 0;
 }
 
-static void ens_set_structure(Ensemble* e, int mask) {
+// A scan or an override of the structure also refreshes the panel copies
+// (the structure is rescanned whenever the matrices change).
+static int ens_set_structure(Ensemble* e, int mask) {
     const int skip = (mask & 0x24) == 0x24;
     if (skip != e->skip_normal) ens_drop_graph(e);
     e->zero_mask = mask;
     e->skip_normal = skip;
+    return ens_update_panels(e);
 }
 
 // Scan the six matrices (owned or borrowed; reads each once) and let the GEMM
@@ -1241,7 +1300,7 @@ int pq3d_ens_scan_structure(void* h, int* mask_out) {
     int mask = 0;
     int rc = scan_zero_matrices(m, 6, MAT_F32, e->ld, e->n, &mask);
     if (rc) return rc;
-    ens_set_structure(e, mask);
+    if ((rc = ens_set_structure(e, mask))) return rc;
     if (mask_out) *mask_out = mask;
     return 0;
 }
@@ -1253,8 +1312,7 @@ int pq3d_ens_set_structure(void* h, int mask) {
     Ensemble* e = (Ensemble*)h;
     PQ_CHECK(e && mask >= 0 && mask < 64, "pq3d_ens_set_structure: bad argument");
     PQ_HIP(hipStreamSynchronize(e->stream));
-    ens_set_structure(e, mask);
-    return 0;
+    return ens_set_structure(e, mask);
 }
 
 int pq3d_ens_get_structure(void* h, int* mask, int* skip_normal) {
@@ -1263,6 +1321,25 @@ int pq3d_ens_get_structure(void* h, int* mask, int* skip_normal) {
     PQ_CHECK(e && mask && skip_normal, "pq3d_ens_get_structure: null argument");
     *mask = e->zero_mask;
     *skip_normal = e->skip_normal;
+    return 0;
+}
+
+// Device memory the ensemble holds: owned matrices, fields, operands, the
+// split GEMM's planes and panel copies, partial sums, control blocks.
+int pq3d_ens_device_bytes(void* h, double* bytes) {
+    clear_error();
+    Ensemble* e = (Ensemble*)h;
+    PQ_CHECK(e && bytes, "pq3d_ens_device_bytes: null argument");
+    const double items = (double)e->n * e->B;
+    double b = (double)E_NFIELDS * items * sizeof(double) + 2.0 * items * sizeof(float) +
+               (double)e->nchunk * 3 * e->B * e->npad * sizeof(float) + 2.0 * e->B * sizeof(MemberCtrl) +
+               (double)ENS_ROUNDS * e->B * 2 * 8 + sizeof(EnsCtrl);
+    for (int k = 0; k < 6; ++k) {
+        if (e->own_mats[k]) b += (double)e->ld * e->n * sizeof(float);
+        if (e->panel[k]) b += (double)ens_panel_bytes(e);
+    }
+    if (e->planes) b += (double)2 * 3 * (e->npad / 8) * 64 * sizeof(uint4);
+    *bytes = b;
     return 0;
 }
 

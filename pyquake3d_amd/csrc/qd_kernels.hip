@@ -943,6 +943,22 @@ int pq3d_dense_attach_hmatrix(void* h, void* hm) {
     return 0;
 }
 
+// Device memory the simulation holds: matrices, fields, partial sums,
+// control block, step records and pore-pressure profiles.
+int pq3d_dense_device_bytes(void* h, double* bytes) {
+    clear_error();
+    DenseSim* s = (DenseSim*)h;
+    PQ_CHECK(s && bytes, "pq3d_dense_device_bytes: null argument");
+    double b = 0;
+    for (int k = 0; k < 6; ++k)
+        if (s->mats[k]) b += (double)s->ld * s->n * mat_elem_bytes(s->mat_type);
+    b += (double)NFIELDS * s->n * sizeof(double) + (double)s->nchunk * 3 * s->n * sizeof(double) + sizeof(Ctrl);
+    b += (double)s->rec_cap * 4 * sizeof(double);
+    if (s->dila_on) b += (double)s->dila.K * s->n * sizeof(double) + 3.0 * (s->dila.K - 1) * sizeof(double);
+    *bytes = b;
+    return 0;
+}
+
 int pq3d_dense_matrix_ptrs(void* h, void** ptrs, long long* ld) {
     clear_error();
     DenseSim* s = (DenseSim*)h;

@@ -28,6 +28,7 @@ def _declare(L):
     _lib._sig(L, "pq3d_ens_create", [ctypes.POINTER(vp), ll, i, ctypes.POINTER(vp), ll, i])
     _lib._sig(L, "pq3d_ens_destroy", [vp])
     _lib._sig(L, "pq3d_ens_matrix_ptrs", [vp, ctypes.POINTER(vp), ctypes.POINTER(ll)])
+    _lib._sig(L, "pq3d_ens_device_bytes", [vp, dp])
     _lib._sig(L, "pq3d_ens_set_field", [vp, i, dp])
     _lib._sig(L, "pq3d_ens_get_field", [vp, i, dp])
     _lib._sig(L, "pq3d_ens_set_member_field", [vp, i, i, dp])
@@ -81,6 +82,12 @@ class Ensemble:
         self.fix_tn = False
         if precision != "fp32":
             self.set_precision(precision)
+
+    def device_bytes(self):
+        """Device memory the ensemble holds (borrowed matrices not counted)."""
+        b = ctypes.c_double()
+        check(_lib.lib().pq3d_ens_device_bytes(self.h, ctypes.byref(b)), "ens device bytes")
+        return b.value
 
     def matrices_changed(self):
         """Mark owned matrices as rewritten through ``mat_ptrs``."""

@@ -59,11 +59,21 @@ class _Mesh:
         self.busy_since = None
 
     def hbm_bytes(self):
-        ld = (self.n + 3) // 4 * 4
-        gf = 6 * ld * self.n * 4
-        ens = (31 * 64 * self.n * 8 + 2 * 64 * self.n * 4 + 32 * 3 * 64 * ((self.n + 63) // 64 * 64) * 4)
-        dense_state = 25 * self.n * 8
-        return gf + ens + dense_state
+        """Device memory the mesh needs when resident, an upper bound of what
+        the library then reports (``DenseSim.device_bytes`` +
+        ``Ensemble.device_bytes``): the FP32 Green's functions, the dense
+        simulation's fields and split-K partials (kc 256), and the ensemble's
+        fields, operands, partials (at most 32 chunks), the split GEMM's BF16
+        operand planes and panel copies of all six matrices."""
+        n = self.n
+        ld = (n + 3) // 4 * 4
+        npad = (n + 63) // 64 * 64
+        kpad = (n + 31) // 32 * 32
+        gf = 6 * ld * n * 4
+        dense = 27 * n * 8 + 3 * n * ((n + 255) // 256) * 8 + 4096
+        ens = (31 * 64 * n * 8 + 2 * 64 * n * 4 + 32 * 3 * 64 * npad * 4 + 2 * 3 * (npad // 8) * 64 * 16
+               + 6 * npad * kpad * 4 + 8192)
+        return gf + dense + ens
 
     @property
     def resident(self):

@@ -67,6 +67,11 @@ def test_admission_under_hbm_budget():
 
     srv._admit = admit
     m = srv.run(_requests(xg, 8, seed=9))
+    # the budget formula bounds what the library reports for a resident mesh
+    for mesh in srv.meshes.values():
+        if mesh.resident:
+            held = mesh.sim.device_bytes() + mesh.ens.device_bytes()
+            assert held <= mesh.hbm_bytes(), (mesh.name, held, mesh.hbm_bytes())
     srv.close()
     assert m["requests"] == 8 and m["failed"] == 0
     assert peak[0] <= budget
