@@ -531,8 +531,11 @@ def main():
     ap.add_argument("--plain-steps", type=int, default=2000)
     ap.add_argument("--graph-steps", type=int, default=8)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--serve-requests", type=int, default=24, help="0 skips the serving leg")
-    ap.add_argument("--serve-rate", type=float, default=1.0, help="Poisson arrival rate, requests/s")
+    ap.add_argument("--serve-requests", type=int, default=400, help="0 skips the serving leg")
+    ap.add_argument("--serve-rate", type=float, default=40.0,
+                    help="Poisson arrival rate, requests/s over the three meshes (40/s keeps the 30k mesh's "
+                         "64 slots more than half full)")
+    ap.add_argument("--serve-only", action="store_true", help="run only the serving leg (JSON of that leg)")
     ap.add_argument("--no-hmat", action="store_true", help="skip the 60k H-matrix leg")
     ap.add_argument("--no-config1", action="store_true", help="skip the 1,600-element 2,000-step leg")
     ap.add_argument("--hmat-only", action="store_true", help="run only the 60k H-matrix leg (JSON of that leg)")
@@ -554,6 +557,10 @@ def main():
             _lib.check(_lib.lib().pq3d_memcpy_d2d(dst.ptr, src.ptr, nbytes), "copy")
         _lib.lib().pq3d_synchronize()
         print(json.dumps({"copy_bytes": nbytes, "copies": 5}))
+        return
+
+    if args.serve_only:
+        print(json.dumps(serve(args.serve_requests, args.serve_rate)))
         return
 
     if args.hmat_only:
@@ -702,7 +709,7 @@ def main():
         serving = serve(args.serve_requests, args.serve_rate)
         log(f"[bench] serving: {serving['requests']} requests, p50 {serving['p50_time_to_result_s']:.2f} s, "
             f"p99 {serving['p99_time_to_result_s']:.2f} s, {serving['member_steps_per_s']:.0f} member-steps/s, "
-            f"occupancy {serving['mean_occupancy']:.2f}")
+            f"occupancy {serving['mean_occupancy']:.2f} (30k mesh {serving['occupancy_per_mesh']['30k']:.2f})")
 
     config1 = None if args.no_config1 else config1_leg(graph_steps=args.graph_steps,
                                                        cpu_budget_s=0.0 if args.no_cpu_baseline else 8.0)

@@ -173,12 +173,14 @@ def cached(key):
     return int(e["kc"]) if isinstance(e, dict) and "kc" in e else None
 
 
-def ensure_ensemble(ens):
-    """Apply the cached split-K width for this shape, tuning once if absent.
-    Returns the kc in use."""
+def ensure_ensemble(ens, tune=True):
+    """Apply the cached split-K width for this shape, tuning once if absent
+    (with ``tune=False`` the built-in width stays when nothing is cached, as
+    when other work would share the device during the timing).  Returns the
+    kc in use."""
     kc = cached(ensemble_key(ens))
     if kc is None:
-        return tune_ensemble(ens)["kc"]
+        return tune_ensemble(ens)["kc"] if tune else ens.kc
     if kc != ens.kc:
         ens.set_kc(kc)
     return ens.kc

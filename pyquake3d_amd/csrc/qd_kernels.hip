@@ -594,7 +594,9 @@ __global__ void pad_zero_kernel(double* p, long long n) {
 }
 
 // ---------------------------------------------------------------------------
-// Host side: the dense simulation object
+// Host side: the dense simulation object.  All of a simulation's work runs on
+// the null stream and its calls wait for that stream only, so ensembles on
+// their own (non-blocking) streams keep running meanwhile.
 // ---------------------------------------------------------------------------
 // ---------------------------------------------------------------------------
 // Dilatancy: pore-pressure diffusion normal to the fault, one backward-Euler
@@ -908,7 +910,7 @@ int pq3d_dense_dilatancy_update(void* h, double dt) {
     hipLaunchKernelGGL(dila_kernel, dim3((unsigned)((s->n + 255) / 256)), dim3(256), 0, 0, s->fields, s->n,
                        (const Ctrl*)s->ctrl, s->dila_parr, (const double*)s->dila_tri, s->dila, dt);
     PQ_HIP(hipGetLastError());
-    PQ_HIP(hipDeviceSynchronize());
+    PQ_HIP(hipStreamSynchronize(0));
     return 0;
 }
 
@@ -977,7 +979,7 @@ int pq3d_dense_set_field(void* h, int field, const double* host) {
         hipLaunchKernelGGL(rake_trig_kernel, dim3((unsigned)((s->n + 255) / 256)), dim3(256), 0, 0, dst,
                            s->fields + (size_t)F_COSR0 * s->n, s->fields + (size_t)F_SINR0 * s->n, s->n);
         PQ_HIP(hipGetLastError());
-        PQ_HIP(hipDeviceSynchronize());
+        PQ_HIP(hipStreamSynchronize(0));
     }
     return 0;
 }
@@ -1069,7 +1071,7 @@ int pq3d_dense_derivative(void* h, const double* d_Tno, const double* d_Tt1o, co
     hipLaunchKernelGGL(rhs_eval_kernel, dim3((unsigned)((s->n + 255) / 256)), dim3(256), 0, 0, a, d_Tno, d_Tt1o,
                        d_Tt2o, d_state, d_out);
     PQ_HIP(hipGetLastError());
-    PQ_HIP(hipDeviceSynchronize());
+    PQ_HIP(hipStreamSynchronize(0));
     return 0;
 }
 
@@ -1134,7 +1136,7 @@ int pq3d_dense_scan_structure(void* h, int* mask_out) {
     PQ_CHECK(s, "pq3d_dense_scan_structure: null handle");
     int mask = 0;
     if (s->mat_type != MAT_NONE) {
-        PQ_HIP(hipDeviceSynchronize());
+        PQ_HIP(hipStreamSynchronize(0));
         int rc = scan_zero_matrices(s->mats, 6, s->mat_type, s->ld, s->n, &mask);
         if (rc) return rc;
     }
@@ -1179,7 +1181,7 @@ int pq3d_dense_matvec(void* h) {
     PQ_CHECK(s->mat_type != MAT_NONE || s->hmat, "dense sim: no matrices and no H-matrix attached");
     launch_product(s, 0);
     PQ_HIP(hipGetLastError());
-    PQ_HIP(hipDeviceSynchronize());
+    PQ_HIP(hipStreamSynchronize(0));
     return 0;
 }
 
@@ -1195,7 +1197,7 @@ int pq3d_dense_set_kc(void* h, int kc) {
     PQ_CHECK(kc >= 1, "pq3d_dense_set_kc: kc must be positive (got %d)", kc);
     const int kcn = (int)std::min<long long>(kc, s->n);
     const int nchunk = (int)((s->n + kcn - 1) / kcn);
-    PQ_HIP(hipDeviceSynchronize());
+    PQ_HIP(hipStreamSynchronize(0));
     if (s->graph) {
         (void)hipGraphExecDestroy(s->graph);
         s->graph = nullptr;
@@ -1274,7 +1276,7 @@ int pq3d_dense_step(void* h, double dttry, double* h_out, double* dtnext_out, in
     PQ_HIP(hipMemcpy(&s->ctrl->dttry, &dttry, sizeof(double), hipMemcpyHostToDevice));
     launch_step(s, 0, nullptr, -1);
     PQ_HIP(hipGetLastError());
-    PQ_HIP(hipDeviceSynchronize());
+    PQ_HIP(hipStreamSynchronize(0));
     Ctrl c;
     PQ_HIP(hipMemcpy(&c, s->ctrl, sizeof(Ctrl), hipMemcpyDeviceToHost));
     *h_out = c.last_h;
@@ -1333,7 +1335,7 @@ int pq3d_dense_run(void* h, long long nsteps, int graph_steps, double* records, 
     }
     for (; done < nsteps; ++done) launch_step(s, st, records ? s->d_records : nullptr, records ? done : -1);
     PQ_HIP(hipGetLastError());
-    PQ_HIP(hipDeviceSynchronize());
+    PQ_HIP(hipStreamSynchronize(0));
     Ctrl c;
     PQ_HIP(hipMemcpy(&c, s->ctrl, sizeof(Ctrl), hipMemcpyDeviceToHost));
     if (records) {

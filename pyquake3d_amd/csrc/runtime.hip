@@ -62,8 +62,11 @@ int scan_zero_matrices(void* const* mats, int nmat, int mat_type, long long ld, 
     if (mat_type == MAT_F32) keep = make_uint4(0x7fffffffu, 0x7fffffffu, 0x7fffffffu, 0x7fffffffu);
     else if (mat_type == MAT_BF16) keep = make_uint4(0x7fff7fffu, 0x7fff7fffu, 0x7fff7fffu, 0x7fff7fffu);
     else keep = make_uint4(0xffffffffu, 0x7fffffffu, 0xffffffffu, 0x7fffffffu);  // low word, high word
+    // flags in a per-thread scratch kept across calls: no hipMalloc / hipFree
+    // (a hipFree waits for the whole device, work of other streams included)
+    static thread_local Scratch flags;
     int* d = nullptr;
-    PQ_HIP(hipMalloc((void**)&d, 8 * sizeof(int)));
+    if (int rc = scratch_get(flags, 8 * sizeof(int), (void**)&d)) return rc;
     hipError_t e = hipMemset(d, 0, 8 * sizeof(int));
     const long long nwords = (long long)(bytes / 16);
     for (int k = 0; k < nmat && e == hipSuccess; ++k) {
@@ -74,7 +77,6 @@ int scan_zero_matrices(void* const* mats, int nmat, int mat_type, long long ld, 
     }
     int h[8] = {0};
     if (e == hipSuccess) e = hipMemcpy(h, d, 8 * sizeof(int), hipMemcpyDeviceToHost);
-    (void)hipFree(d);
     PQ_HIP(e);
     *mask = 0;
     for (int k = 0; k < nmat; ++k)

@@ -262,7 +262,7 @@ static int gf_assemble_staged(const GfAssembleArgs& a, char* base) {
                                a.rows ? d_rows : nullptr, nr, nc, spb, a.mu, a.lam, nu, a.halfspace, o, a.ld);
         }
         e = hipGetLastError();
-        if (e == hipSuccess) e = hipDeviceSynchronize();
+        if (e == hipSuccess) e = hipStreamSynchronize(0);  // the null stream only: other streams keep running
     }
     if (e != hipSuccess) {
         set_error("pq3d_gf_assemble: %s", hipGetErrorString(e));

@@ -3,7 +3,9 @@
 * A request's result (simulated time, state, slip) is bit-identical whether it
   runs alone or admitted among others arriving at random times on two meshes.
 * With an HBM budget that holds one mesh at a time, requests for both meshes
-  all complete and the resident footprint never exceeds the budget.
+  all complete and the resident footprint (meshes resident or loading) never
+  exceeds the budget; the budget formula bounds the memory the library
+  reports for a resident mesh.
 """
 import numpy as np
 import pytest
@@ -59,13 +61,13 @@ def test_admission_under_hbm_budget():
     budget = max(sizes.values()) * 1.05   # one mesh at a time
     srv = Server(meshes, budget, steps_per_iteration=16, graph_steps=8)
     peak = [0]
-    orig = srv._admit
+    orig = srv._start_loads
 
-    def admit(now):
-        orig(now)
+    def start_loads():
+        orig()
         peak[0] = max(peak[0], srv._resident_bytes())
 
-    srv._admit = admit
+    srv._start_loads = start_loads
     m = srv.run(_requests(xg, 8, seed=9))
     # the budget formula bounds what the library reports for a resident mesh
     for mesh in srv.meshes.values():
