@@ -318,10 +318,10 @@ __device__ __forceinline__ void ens_gemm3_half(const float4 (&a)[NM][8], const b
                                                v4f (&acc)[NM][4][4]) {
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
-        // subtile i of the half's matrices: element t of a fragment = source 8 (lane / 16) + t
-        bf8 f[NM][3];
 #pragma unroll
         for (int q = 0; q < NM; ++q) {
+            // subtile i of matrix q: element t of a fragment = source 8 (lane / 16) + t
+            bf8 f[3];
             unsigned h[4], m[4], l[4];
 #pragma unroll
             for (int t = 0; t < 4; ++t) {
@@ -335,21 +335,19 @@ __device__ __forceinline__ void ens_gemm3_half(const float4 (&a)[NM][8], const b
             }
             const uint4 hv = make_uint4(h[0], h[1], h[2], h[3]), mv = make_uint4(m[0], m[1], m[2], m[3]),
                         lv = make_uint4(l[0], l[1], l[2], l[3]);
-            f[q][0] = *reinterpret_cast<const bf8*>(&hv);
-            f[q][1] = *reinterpret_cast<const bf8*>(&mv);
-            f[q][2] = *reinterpret_cast<const bf8*>(&lv);
-        }
-        // six products hi.hi, hi.mid, mid.hi, hi.lo, lo.hi, mid.mid; 2 L
-        // independent accumulators between dependent MFMAs
+            f[0] = *reinterpret_cast<const bf8*>(&hv);
+            f[1] = *reinterpret_cast<const bf8*>(&mv);
+            f[2] = *reinterpret_cast<const bf8*>(&lv);
+            // six products hi.hi, hi.mid, mid.hi, hi.lo, lo.hi, mid.mid; L
+            // independent accumulators between dependent MFMAs
 #pragma unroll
-        for (int p = 0; p < 6; ++p) {
-            const int pa = (p == 0 || p == 1 || p == 3) ? 0 : (p == 2 || p == 5) ? 1 : 2;
-            const int pu = (p == 0 || p == 2 || p == 4) ? 0 : (p == 1 || p == 5) ? 1 : 2;
-#pragma unroll
-            for (int q = 0; q < NM; ++q)
+            for (int p = 0; p < 6; ++p) {
+                const int pa = (p == 0 || p == 1 || p == 3) ? 0 : (p == 2 || p == 5) ? 1 : 2;
+                const int pu = (p == 0 || p == 2 || p == 4) ? 0 : (p == 1 || p == 5) ? 1 : 2;
 #pragma unroll
                 for (int j = 0; j < L; ++j)
-                    acc[q][i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(f[q][pa], uf[j][pu], acc[q][i][j], 0, 0, 0);
+                    acc[q][i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(f[pa], uf[j][pu], acc[q][i][j], 0, 0, 0);
+            }
         }
     }
 }
@@ -418,17 +416,15 @@ __device__ __forceinline__ void ens_gemm3_tile(const float* const (&S)[NM], cons
     load_a(ad, 1, k0);
     load_u(ud, 1, k0);
     for (long long kb = k0; kb < k1; kb += 32) {
-        const bool more = kb + 32 < k1;
+        // the next step's block (the last step reloads its own, from cache):
+        // unconditional loads keep each buffer in the same registers
+        const long long kn = kb + 32 < k1 ? kb + 32 : kb;
         ens_gemm3_half<L, NM>(as, us, acc);   // strike matrices x u1
-        if (more) {
-            load_a(as, 0, kb + 32);
-            load_u(us, 0, kb + 32);
-        }
+        load_a(as, 0, kn);
+        load_u(us, 0, kn);
         ens_gemm3_half<L, NM>(ad, ud, acc);   // dip matrices x u2 into the same components
-        if (more) {
-            load_a(ad, 1, kb + 32);
-            load_u(ud, 1, kb + 32);
-        }
+        load_a(ad, 1, kn);
+        load_u(ud, 1, kn);
     }
     // C fragment as in ens_gemm_tile
 #pragma unroll
