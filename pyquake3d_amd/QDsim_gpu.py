@@ -577,67 +577,71 @@ class QDsim:
         self.state = np.log(np.sinh(self.Tt / (self.Tno - self.P * 1e-6) / self.a) * 2.0 * self.V0 / self.slipv) * self.a
 
     def Init_condition(self):
-        """Initial stress, friction parameters and state (QDsim_gpu.py:319-609)."""
+        """Initial stress, friction parameters and state (QDsim_gpu.py:319-609):
+        tractions from a depth-scaled principal stress state (or the constant
+        vertical stress), then either the per-element parameter file or the
+        zone rules of ``_zone_init``."""
         P = self.Para0
         N = len(self.eleVec)
-        self.Tt1o = np.zeros(N)
-        self.Tt2o = np.zeros(N)
-        self.Tno = np.zeros(N)
-        self.fix_Tn = P["Fix_Tn"] == "True"
-        ssv_scale = float(P["ssv_scale"])
-        ssh1_scale = float(P["ssh1_scale"])
-        ssv0_scale = float(P["ssh2_scale"])
-        trac_nor = float(P["Vertical principal stress value"])
+        flag = lambda key: P[key] == "True"  # noqa: E731
+        self.fix_Tn = flag("Fix_Tn")
+        scale_v = float(P["ssv_scale"])
+        scale_h1 = float(P["ssh1_scale"])
+        scale_h2 = float(P["ssh2_scale"])
+        sigma_v0 = float(P["Vertical principal stress value"])
         self.P0 = 0
         self.P = np.zeros(N)
         self.dPdt0 = np.zeros(N)
         self.local_index = np.arange(0, N, 1)
         if self.Ifdila:
             # pore-pressure grid and profiles (QDsim_gpu.py:336-349), pressures in Pa
-            num, c = 60, 1e-2
+            nodes, stretch = 60, 1e-2
             self.P0 = float(P["Constant porepressure"]) * 1e6
             self.P = np.ones(N) * float(P["Initial porepressure"]) * 1e6
-            self.yp = np.logspace(start=-3, stop=10, num=num, base=10)
-            self.zp = np.log(1 + self.yp / c)
-            self.Parr = np.ones([N, num]) * float(P["Initial porepressure"]) * 1e6
-        solve_normal = P["Normal traction solved from stress tensor"] == "True"
-        depth_var = P["Vertical principal stress value varies with depth"] == "True"
-        Ph1ang = np.pi / 180. * float(P["Angle between ssh1 and X-axis"])
-        Rm = np.array([[np.cos(Ph1ang), -np.sin(Ph1ang)], [np.sin(Ph1ang), np.cos(Ph1ang)]])
-        for i in range(N):
-            if depth_var:
-                turning_dep = float(P["Turnning depth"])
-                ssv = -self.xg[i, 2] / turning_dep + 0.2
-                if ssv > 1.0:
-                    ssv = 1.0
-                ssv = trac_nor * ssv * ssv_scale
-            else:
-                ssv = trac_nor * ssv_scale
-            ssh1 = -ssv * ssh1_scale
-            ssh2 = -ssv * ssv0_scale
-            ssv = -ssv
-            stress = np.dot(np.dot(Rm, np.array([[ssh1, 0], [0, ssh2]])), Rm.transpose())
-            stress3D = np.array([[stress[0][0], stress[0][1], 0], [stress[1][0], stress[1][1], 0], [0, 0, ssv]])
-            tra = np.dot(stress3D, self.eleVec[i, -3:])
-            ev = self.eleVec[i]
-            self.Tt1o[i] = tra[0] * ev[0] + tra[1] * ev[1] + tra[2] * ev[2]
-            self.Tt2o[i] = tra[0] * ev[3] + tra[1] * ev[4] + tra[2] * ev[5]
-            self.Tno[i] = tra[0] * ev[6] + tra[1] * ev[7] + tra[2] * ev[8]
-            if not solve_normal:
-                self.Tno[i] = ssv
+            self.yp = np.logspace(start=-3, stop=10, num=nodes, base=10)
+            self.zp = np.log(1 + self.yp / stretch)
+            self.Parr = np.ones([N, nodes]) * float(P["Initial porepressure"]) * 1e6
+        normal_from_tensor = flag("Normal traction solved from stress tensor")
+        depth_scaled = flag("Vertical principal stress value varies with depth")
+        # vertical principal stress per element: constant, or growing linearly
+        # with depth down to the turning depth (0.2 at the surface, 1 below)
+        if depth_scaled:
+            ramp = np.minimum(-self.xg[:, 2] / float(P["Turnning depth"]) + 0.2, 1.0)
+            sv = sigma_v0 * ramp * scale_v
+        else:
+            sv = np.full(N, sigma_v0 * scale_v)
+        # compressive principal stresses; the horizontal pair rotated by the
+        # angle of the first one from the x axis: S = R diag(h1, h2) R^T
+        phi = np.pi / 180. * float(P["Angle between ssh1 and X-axis"])
+        c, s = np.cos(phi), np.sin(phi)
+        h1, h2, vz = -sv * scale_h1, -sv * scale_h2, -sv
+        Sxx = (c * h1) * c + (-s * h2) * -s
+        Sxy = (c * h1) * s + (-s * h2) * c
+        Syx = (s * h1) * c + (c * h2) * -s
+        Syy = (s * h1) * s + (c * h2) * c
+        nrm = self.eleVec[:, 6:9]
+        t = np.stack([Sxx * nrm[:, 0] + Sxy * nrm[:, 1] + 0.0 * nrm[:, 2],
+                      Syx * nrm[:, 0] + Syy * nrm[:, 1] + 0.0 * nrm[:, 2],
+                      0.0 * nrm[:, 0] + 0.0 * nrm[:, 1] + vz * nrm[:, 2]], axis=1)
+        ev = self.eleVec
+        self.Tt1o = t[:, 0] * ev[:, 0] + t[:, 1] * ev[:, 1] + t[:, 2] * ev[:, 2]
+        self.Tt2o = t[:, 0] * ev[:, 3] + t[:, 1] * ev[:, 4] + t[:, 2] * ev[:, 5]
+        self.Tno = t[:, 0] * ev[:, 6] + t[:, 1] * ev[:, 7] + t[:, 2] * ev[:, 8]
+        if not normal_from_tensor:
+            self.Tno = vz.copy()
         self.Tno = np.abs(self.Tno)
         self.Tt = np.sqrt(self.Tt1o * self.Tt1o + self.Tt2o * self.Tt2o)
         with np.errstate(invalid="ignore", divide="ignore"):
-            x = self.Tt1o / self.Tt
-            y = self.Tt2o / self.Tt
-        solve_shear = P["Shear traction solved from stress tensor"] == "True"
-        if P["Rake solved from stress tensor"] == "True":
-            self.rake = np.arctan2(y, x)
+            cos_r = self.Tt1o / self.Tt
+            sin_r = self.Tt2o / self.Tt
+        shear_from_tensor = flag("Shear traction solved from stress tensor")
+        if flag("Rake solved from stress tensor"):
+            self.rake = np.arctan2(sin_r, cos_r)
         else:
-            self.rake = np.ones(len(self.Tt)) * float(P["Fix_rake"]) / 180.0 * np.pi
-        if not solve_normal and not depth_var:
-            self.Tno = np.ones(len(self.Tno)) * trac_nor
-        self.vec_Tra = np.array([x, y]).transpose()
+            self.rake = np.ones(N) * float(P["Fix_rake"]) / 180.0 * np.pi
+        if not normal_from_tensor and not depth_scaled:
+            self.Tno = np.ones(N) * sigma_v0
+        self.vec_Tra = np.array([cos_r, sin_r]).transpose()
         self.Vpl_con = float(P["Plate loading rate"])
         self.slipvC = np.ones(len(self.xg)) * self.Vpl_con
         self.shear_loading = np.zeros(N)
@@ -662,66 +666,71 @@ class QDsim:
             self.rake0 = np.copy(self.rake)
             self.calc_nucleaszie_cohesivezone()
             return
-        self._zone_init(solve_shear)
+        self._zone_init(shear_from_tensor)
 
-    def _zone_init(self, solve_shear):
-        """Zone-based friction parameters (QDsim_gpu.py:473-609)."""
+    @staticmethod
+    def _nearest(points, targets, chunk=4096):
+        """Distance from every point to the nearest target (sqrt of the least
+        squared distance: the same value as the least distance)."""
+        out = np.empty(len(points))
+        for s in range(0, len(points), chunk):
+            d = points[s:s + chunk, None, :] - targets[None, :, :]
+            out[s:s + chunk] = np.sqrt(np.min(np.sum(d * d, axis=2), axis=1))
+        return out
+
+    def _zone_init(self, shear_from_tensor):
+        """Friction zones from the distance to the mesh boundary
+        (QDsim_gpu.py:473-609): velocity-strengthening within a band along the
+        buried edges (and a narrower one along the free surface), a linear
+        transition band, velocity-weakening inside; an optional nucleation
+        disc; then steady-state traction and state at the initial slip rate."""
         P = self.Para0
-        N = len(self.eleVec)
-        _, boundary_nodes = find_boundary_edges_and_nodes(self.elelst)
-        boundary_coord = self.nodelst[boundary_nodes - 1]
-        index_surface = np.where(np.abs(boundary_coord[:, 2] - 0.0) < 1e-5)[0]
-        index_sb = np.setdiff1d(np.arange(0, len(boundary_coord), 1), index_surface)
-        bsurf = boundary_coord[index_surface]
-        bsb = boundary_coord[index_sb]
-        nuclearloc = np.array([float(P["Nuclea_posx"]), float(P["Nuclea_posy"]), float(P["Nuclea_posz"])])
-        Wedge = float(P["Widths of VS region"])
-        Wedge_surface = float(P["Widths of surface VS region"])
-        transregion = float(P["Transition region from VS to VW region"])
-        aVs, bVs, dcVs = (float(P[k]) for k in ("Rate-and-state parameters a in VS region",
-                                                "Rate-and-state parameters b in VS region",
-                                                "Characteristic slip distance in VS region"))
-        aVw, bVw, dcVw = (float(P[k]) for k in ("Rate-and-state parameters a in VW region",
-                                                "Rate-and-state parameters b in VW region",
-                                                "Characteristic slip distance in VW region"))
-        aNu, bNu, dcNu = (float(P[k]) for k in ("Rate-and-state parameters a in nucleation region",
-                                                "Rate-and-state parameters b in nucleation region",
-                                                "Characteristic slip distance in nucleation region"))
-        slivpNu = float(P["Initial slip rate in nucleation region"])
-        Set_nuclear = P["Set_nucleation"] == "True"
-        Radiu_nuclear = float(P["Radius of nucleation"])
-        ChangefriA = P["ChangefriA"] == "True"
-        T_globalarr = []
-        for i in range(N):
-            distem = np.min(np.linalg.norm(bsb - self.xg[i], axis=1))
-            dis = distem / Wedge
-            dis_surface = dis
-            dis1 = (distem - Wedge) / transregion
-            if len(bsurf) > 10:
-                distem_surface = np.min(np.linalg.norm(bsurf - self.xg[i], axis=1))
-                dis_surface = distem_surface / Wedge_surface
-                dis1 = min(distem - Wedge, distem_surface - Wedge_surface) / transregion
-            if dis < 1.0 or dis_surface < 1.0:
-                self.a[i], self.b[i], self.dc[i] = aVs, bVs, dcVs
-            elif dis1 < 1.0:
-                if ChangefriA:
-                    self.a[i] = aVs - (aVs - aVw) * dis1
-                    self.b[i] = bVs
-                else:
-                    self.a[i] = aVs
-                    self.b[i] = bVs - (bVs - bVw) * dis1
-                self.dc[i] = dcVs
-            else:
-                self.a[i], self.b[i], self.dc[i] = aVw, bVw, dcVw
-            if np.linalg.norm(self.xg[i] - nuclearloc) < Radiu_nuclear and Set_nuclear:
-                self.slipv[i] = slivpNu
-                self.dc[i], self.a[i], self.b[i] = dcNu, aNu, bNu
-            T_local = np.array([np.cos(self.rake[i]), np.sin(self.rake[i]), 0.0])
-            T_globalarr.append(np.dot(self.eleVec[i].reshape([3, 3]).transpose(), T_local))
-        self.T_globalarr = np.array(T_globalarr)
+        num = lambda key: float(P[key])  # noqa: E731
+        _, edge_nodes = find_boundary_edges_and_nodes(self.elelst)
+        edge_xyz = self.nodelst[edge_nodes - 1]
+        at_surface = np.abs(edge_xyz[:, 2] - 0.0) < 1e-5
+        buried = edge_xyz[~at_surface]
+        surface = edge_xyz[at_surface]
+        band = num("Widths of VS region")
+        band_surface = num("Widths of surface VS region")
+        ramp = num("Transition region from VS to VW region")
+        vs = [num(f"Rate-and-state parameters {k} in VS region") for k in ("a", "b")] + \
+            [num("Characteristic slip distance in VS region")]
+        vw = [num(f"Rate-and-state parameters {k} in VW region") for k in ("a", "b")] + \
+            [num("Characteristic slip distance in VW region")]
+        nuc = [num(f"Rate-and-state parameters {k} in nucleation region") for k in ("a", "b")] + \
+            [num("Characteristic slip distance in nucleation region")]
+        d_edge = self._nearest(self.xg, buried)
+        u_edge = d_edge / band                    # < 1 inside the buried-edge band
+        u_surf = u_edge
+        u_ramp = (d_edge - band) / ramp
+        if len(surface) > 10:
+            d_surf = self._nearest(self.xg, surface)
+            u_surf = d_surf / band_surface
+            u_ramp = np.minimum(d_edge - band, d_surf - band_surface) / ramp
+        in_vs = (u_edge < 1.0) | (u_surf < 1.0)
+        in_ramp = ~in_vs & (u_ramp < 1.0)
+        in_vw = ~in_vs & ~in_ramp
+        a_ramp_mode = P["ChangefriA"] == "True"   # the transition tapers a (else b)
+        self.a = np.where(in_vs, vs[0], np.where(in_vw, vw[0], vs[0] - (vs[0] - vw[0]) * u_ramp if a_ramp_mode
+                                                  else vs[0]))
+        self.b = np.where(in_vs, vs[1], np.where(in_vw, vw[1], vs[1] if a_ramp_mode
+                                                  else vs[1] - (vs[1] - vw[1]) * u_ramp))
+        self.dc = np.where(in_vw, vw[2], vs[2])
+        if P["Set_nucleation"] == "True":
+            centre = np.array([num("Nuclea_posx"), num("Nuclea_posy"), num("Nuclea_posz")])
+            disc = np.linalg.norm(self.xg - centre, axis=1) < num("Radius of nucleation")
+            self.slipv[disc] = num("Initial slip rate in nucleation region")
+            self.dc = np.where(disc, nuc[2], self.dc)
+            self.a = np.where(disc, nuc[0], self.a)
+            self.b = np.where(disc, nuc[1], self.b)
+        # slip direction in global coordinates: frame^T (cos rake, sin rake, 0)
+        frames = self.eleVec.reshape(-1, 3, 3)
+        slip_dir = np.stack([np.cos(self.rake), np.sin(self.rake), np.zeros(len(self.rake))], axis=1)
+        self.T_globalarr = np.einsum("nki,nk->ni", frames, slip_dir)
         self.slipv1 = self.slipv * np.cos(self.rake) + 1e-16
         self.slipv2 = self.slipv * np.sin(self.rake) + 1e-16
-        if not solve_shear:
+        if not shear_from_tensor:
             self.Tt = self.Tno * self.a * np.arcsinh(
                 self.slipv / (2.0 * self.V0) * np.exp((self.f0 + self.b * np.log(self.V0 / self.slipvC)) / self.a))
         self.vec_Tra = np.array([np.cos(self.rake), np.sin(self.rake)]).transpose()
