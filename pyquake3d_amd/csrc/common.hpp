@@ -8,6 +8,8 @@
 #include <cstdio>
 #include <cstring>
 #include <string>
+#include <tuple>
+#include <utility>
 
 namespace pq3d {
 
@@ -31,6 +33,49 @@ void clear_error();
             return -2;                   \
         }                                \
     } while (0)
+
+// Enqueues kernels on a stream, or, when graph is set, appends them to that
+// graph as a chain of kernel nodes (each depending on the one before).  Graphs
+// are built node by node rather than by stream capture: a capture can be
+// invalidated by synchronous HIP calls that other host threads make while it
+// is open (e.g. a mesh loading on one thread while another starts a run).
+struct Launch {
+    hipStream_t stream = 0;
+    hipGraph_t graph = nullptr;
+    hipGraphNode_t last = nullptr;  // tail of the chain
+    hipError_t err = hipSuccess;    // first failure, if any
+
+    explicit Launch(hipStream_t s = 0, hipGraph_t g = nullptr) : stream(s), graph(g) {}
+
+    template <typename... P, typename... A>
+    void operator()(void (*kernel)(P...), dim3 grid, dim3 block, A&&... args) {
+        static_assert(sizeof...(P) == sizeof...(A), "kernel argument count");
+        std::tuple<P...> vals(std::forward<A>(args)...);  // each argument as the kernel's parameter type
+        put(reinterpret_cast<const void*>(kernel), grid, block, vals, std::index_sequence_for<P...>{});
+    }
+
+  private:
+    template <typename Tup, size_t... I>
+    void put(const void* f, dim3 grid, dim3 block, Tup& vals, std::index_sequence<I...>) {
+        void* argv[sizeof...(I) + 1] = {static_cast<void*>(&std::get<I>(vals))..., nullptr};
+        hipError_t e;
+        if (!graph) {
+            e = hipLaunchKernel(f, grid, block, argv, 0, stream);
+        } else {
+            hipKernelNodeParams p{};
+            p.func = const_cast<void*>(f);
+            p.gridDim = grid;
+            p.blockDim = block;
+            p.sharedMemBytes = 0;
+            p.kernelParams = argv;  // copied into the node
+            p.extra = nullptr;
+            hipGraphNode_t node = nullptr;
+            e = hipGraphAddKernelNode(&node, graph, last ? &last : nullptr, last ? 1 : 0, &p);
+            if (e == hipSuccess) last = node;
+        }
+        if (e != hipSuccess && err == hipSuccess) err = e;
+    }
+};
 
 // Grow-only device scratch owned by the calling thread (used by the drop-in
 // entry points whose callers pass host buffers).

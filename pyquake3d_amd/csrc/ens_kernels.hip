@@ -874,13 +874,12 @@ static EnsArgs ens_args(Ensemble* e) {
     return a;
 }
 
-static void ens_launch_prep(Ensemble* e, hipStream_t st) {
+static void ens_launch_prep(Ensemble* e, Launch& L) {
     const long long items = e->n * e->B;
     double* f = e->f;
-    hipLaunchKernelGGL(ens_prep_u_kernel, dim3((unsigned)((e->n + 63) / 64)), dim3(256), 0, st,
-                       f + (size_t)E_SLIPV1 * items, f + (size_t)E_SLIPV2 * items, f + (size_t)E_COSR0 * items,
-                       f + (size_t)E_SINR0 * items, e->Vpl, e->n, e->B, e->mc, e->ctrl, e->U,
-                       e->U + (size_t)e->n * e->B, e->prec ? e->planes : nullptr, e->npad / 8 * 64);
+    L(ens_prep_u_kernel, dim3((unsigned)((e->n + 63) / 64)), dim3(256), f + (size_t)E_SLIPV1 * items,
+      f + (size_t)E_SLIPV2 * items, f + (size_t)E_COSR0 * items, f + (size_t)E_SINR0 * items, e->Vpl, e->n, e->B,
+      e->mc, e->ctrl, e->U, e->U + (size_t)e->n * e->B, e->prec ? e->planes : nullptr, e->npad / 8 * 64);
 }
 
 // the GEMM leaves out Bs and Bd (both zero, or fix_Tn)
@@ -917,38 +916,37 @@ static int ens_update_panels(Ensemble* e) {
 }
 
 // the GEMM alone, on the operands of the last ens_launch_prep
-static void ens_launch_gemm_only(Ensemble* e, hipStream_t st) {
+static void ens_launch_gemm_only(Ensemble* e, Launch& L) {
     const float* U1 = e->U;
     const float* U2 = e->U + (size_t)e->n * e->B;
     dim3 grid((unsigned)((e->n + 63) / 64), (unsigned)e->nchunk);
     if (ens_split_active(e)) {
         const int with_normal = ens_no_normal(e) ? 0 : 1;
         dim3 g3((unsigned)((e->n + 63) / 64), (unsigned)e->nchunk, (unsigned)(1 + with_normal));
-        hipLaunchKernelGGL(ens_gemm3_kernel, g3, dim3(64), 0, st, e->panel[0], e->panel[1], e->panel[2],
-                           e->panel[3], e->panel[4], e->panel[5], e->KB, e->n, e->kc, e->planes, e->npad / 8 * 64,
-                           e->partial, e->npad, e->mc, with_normal);
+        L(ens_gemm3_kernel, g3, dim3(64), e->panel[0], e->panel[1], e->panel[2], e->panel[3], e->panel[4],
+          e->panel[5], e->KB, e->n, e->kc, e->planes, e->npad / 8 * 64, e->partial, e->npad, e->mc, with_normal);
     } else if (ens_no_normal(e))
-        hipLaunchKernelGGL(ens_gemm_kernel<2>, grid, dim3(64), 0, st, e->mats[0], e->mats[1], e->mats[2], e->mats[3],
-                           e->mats[4], e->mats[5], e->ld, e->n, e->kc, U1, U2, e->partial, e->npad, e->mc);
+        L(ens_gemm_kernel<2>, grid, dim3(64), e->mats[0], e->mats[1], e->mats[2], e->mats[3], e->mats[4],
+          e->mats[5], e->ld, e->n, e->kc, U1, U2, e->partial, e->npad, e->mc);
     else
-        hipLaunchKernelGGL(ens_gemm_kernel<3>, grid, dim3(64), 0, st, e->mats[0], e->mats[1], e->mats[2], e->mats[3],
-                           e->mats[4], e->mats[5], e->ld, e->n, e->kc, U1, U2, e->partial, e->npad, e->mc);
+        L(ens_gemm_kernel<3>, grid, dim3(64), e->mats[0], e->mats[1], e->mats[2], e->mats[3], e->mats[4],
+          e->mats[5], e->ld, e->n, e->kc, U1, U2, e->partial, e->npad, e->mc);
 }
 
-static void ens_launch_gemm(Ensemble* e, hipStream_t st) {
-    ens_launch_prep(e, st);
-    ens_launch_gemm_only(e, st);
+static void ens_launch_gemm(Ensemble* e, Launch& L) {
+    ens_launch_prep(e, L);
+    ens_launch_gemm_only(e, L);
 }
 
-static void ens_launch_step(Ensemble* e, hipStream_t st) {
-    ens_launch_gemm(e, st);
+static void ens_launch_step(Ensemble* e, Launch& L) {
+    ens_launch_gemm(e, L);
     EnsArgs a = ens_args(e);
     const unsigned blocks = (unsigned)((e->n * e->B + 255) / 256);
-    hipLaunchKernelGGL(ens_rates_kernel, dim3(blocks), dim3(256), 0, st, a);
+    L(ens_rates_kernel, dim3(blocks), dim3(256), a);
     a.rates_ready = 1;
     for (int r = 0; r < ENS_ROUNDS; ++r) {
-        hipLaunchKernelGGL(ens_attempt_kernel, dim3(blocks), dim3(256), 0, st, a, r);
-        hipLaunchKernelGGL(ens_decide_kernel, dim3(blocks), dim3(256), 0, st, a, r);
+        L(ens_attempt_kernel, dim3(blocks), dim3(256), a, r);
+        L(ens_decide_kernel, dim3(blocks), dim3(256), a, r);
     }
 }
 
@@ -1200,10 +1198,12 @@ int pq3d_ens_gemm(void* h, int reps, float* ms_per) {
     hipEvent_t a, b;
     PQ_HIP(hipEventCreate(&a));
     PQ_HIP(hipEventCreate(&b));
-    ens_launch_gemm(e, e->stream);
+    Launch L(e->stream);
+    ens_launch_gemm(e, L);
     PQ_HIP(hipEventRecord(a, e->stream));
-    for (int r = 0; r < reps; ++r) ens_launch_gemm_only(e, e->stream);
+    for (int r = 0; r < reps; ++r) ens_launch_gemm_only(e, L);
     PQ_HIP(hipEventRecord(b, e->stream));
+    PQ_HIP(L.err);
     PQ_HIP(hipEventSynchronize(b));
     float ms = 0;
     PQ_HIP(hipEventElapsedTime(&ms, a, b));
@@ -1391,9 +1391,10 @@ int pq3d_ens_time_product(void* h, int reps, float* ms_per) {
     if (err == hipSuccess) err = hipEventCreate(&b);
     float ms = 0;
     if (err == hipSuccess) {
+        Launch L(e->stream);
         for (int r = -1; r < reps; ++r) {  // r = -1 is the untimed warm-up
             if (r == 0) (void)hipEventRecord(a, e->stream);
-            ens_launch_gemm(e, e->stream);
+            ens_launch_gemm(e, L);
             hipLaunchKernelGGL(ens_sum_partials_kernel, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0,
                                e->stream, e->partial, e->nchunk, e->B, e->n, e->npad, d);
         }
@@ -1419,21 +1420,21 @@ int pq3d_ens_run(void* h, long long nsteps, int graph_steps, int async) {
     if (nsteps >= graph_steps) {
         if (!e->graph || e->graph_steps != graph_steps) {
             ens_drop_graph(e);
-            hipStream_t cs;
-            PQ_HIP(hipStreamCreateWithFlags(&cs, hipStreamNonBlocking));
-            PQ_HIP(hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal));
-            for (int k = 0; k < graph_steps; ++k) ens_launch_step(e, cs);
             hipGraph_t g;
-            PQ_HIP(hipStreamEndCapture(cs, &g));
-            PQ_HIP(hipGraphInstantiate(&e->graph, g, nullptr, nullptr, 0));
+            PQ_HIP(hipGraphCreate(&g, 0));
+            Launch G(e->stream, g);
+            for (int k = 0; k < graph_steps; ++k) ens_launch_step(e, G);
+            hipError_t err = G.err;
+            if (err == hipSuccess) err = hipGraphInstantiate(&e->graph, g, nullptr, nullptr, 0);
             (void)hipGraphDestroy(g);
-            (void)hipStreamDestroy(cs);
+            PQ_HIP(err);
             e->graph_steps = graph_steps;
         }
         for (; done + graph_steps <= nsteps; done += graph_steps) PQ_HIP(hipGraphLaunch(e->graph, e->stream));
     }
-    for (; done < nsteps; ++done) ens_launch_step(e, e->stream);
-    PQ_HIP(hipGetLastError());
+    Launch L(e->stream);
+    for (; done < nsteps; ++done) ens_launch_step(e, L);
+    PQ_HIP(L.err);
     if (!async) PQ_HIP(hipStreamSynchronize(e->stream));
     return 0;
 }

@@ -372,28 +372,26 @@ template <> unsigned short conv<unsigned short>(double x) { return f2bf((float)x
 
 template <class T>
 static void launch_mvm_t(HMat* h, const double* V1, const double* V2, const double* c0, const double* s0,
-                         double Vpl, int fix_tn, double* out, hipStream_t st) {
+                         double Vpl, int fix_tn, double* out, Launch& L) {
     const long long n = h->n;
-    hipLaunchKernelGGL(hmat_gather_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, h->perm, n, V1, V2,
-                       c0, s0, Vpl, h->xp);
+    L(hmat_gather_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), h->perm, n, V1, V2, c0, s0, Vpl, h->xp);
     if (h->na)
-        hipLaunchKernelGGL(hmat_phase_a_kernel<T>, dim3((unsigned)((h->na + 3) / 4)), dim3(256), 0, st, h->a_items,
-                           h->na, (const T*)h->vdata, h->xp, n, h->t, fix_tn);
+        L(hmat_phase_a_kernel<T>, dim3((unsigned)((h->na + 3) / 4)), dim3(256), h->a_items, h->na,
+          (const T*)h->vdata, h->xp, n, h->t, fix_tn);
     if (h->nbw)
-        hipLaunchKernelGGL(hmat_phase_b_kernel<T>, dim3((unsigned)((h->nbw + 3) / 4)), dim3(256), 0, st,
-                           h->b_waves, h->nbw, h->b_items, h->ddata, (const T*)h->udata, h->xp, n, h->t, h->partial,
-                           fix_tn);
-    hipLaunchKernelGGL(hmat_reduce_kernel, dim3((unsigned)((8 * n + 255) / 256)), dim3(256), 0, st, h->segs,
-                       h->row_seg, h->perm, n, h->partial, out);
+        L(hmat_phase_b_kernel<T>, dim3((unsigned)((h->nbw + 3) / 4)), dim3(256), h->b_waves, h->nbw, h->b_items,
+          h->ddata, (const T*)h->udata, h->xp, n, h->t, h->partial, fix_tn);
+    L(hmat_reduce_kernel, dim3((unsigned)((8 * n + 255) / 256)), dim3(256), h->segs, h->row_seg, h->perm, n,
+      h->partial, out);
 }
 
 // Traction rates of one step into out (3 x n, element order): x1 = V1 - Vpl
 // cos(rake0), x2 = V2 - Vpl sin(rake0) (c0/s0 null: x = V as given).
 void hmat_launch(void* hm, const double* V1, const double* V2, const double* c0, const double* s0, double Vpl,
-                 int fix_tn, double* out, hipStream_t st) {
+                 int fix_tn, double* out, Launch& L) {
     HMat* h = (HMat*)hm;
-    if (h->ftype == MAT_BF16) launch_mvm_t<unsigned short>(h, V1, V2, c0, s0, Vpl, fix_tn, out, st);
-    else launch_mvm_t<float>(h, V1, V2, c0, s0, Vpl, fix_tn, out, st);
+    if (h->ftype == MAT_BF16) launch_mvm_t<unsigned short>(h, V1, V2, c0, s0, Vpl, fix_tn, out, L);
+    else launch_mvm_t<float>(h, V1, V2, c0, s0, Vpl, fix_tn, out, L);
 }
 
 long long hmat_n(void* hm) { return ((HMat*)hm)->n; }
@@ -740,8 +738,9 @@ int pq3d_hmat_mvm(void* hm, const double* x1, const double* x2, int fix_tn, doub
     const long long n = h->n;
     PQ_HIP(hipMemcpy(h->xin, x1, n * sizeof(double), hipMemcpyHostToDevice));
     PQ_HIP(hipMemcpy(h->xin + n, x2, n * sizeof(double), hipMemcpyHostToDevice));
-    hmat_launch(h, h->xin, h->xin + n, nullptr, nullptr, 0.0, fix_tn, h->yout, 0);
-    PQ_HIP(hipGetLastError());
+    Launch L;
+    hmat_launch(h, h->xin, h->xin + n, nullptr, nullptr, 0.0, fix_tn, h->yout, L);
+    PQ_HIP(L.err);
     PQ_HIP(hipMemcpy(y, h->yout, 3 * n * sizeof(double), hipMemcpyDeviceToHost));
     return 0;
 }
@@ -755,10 +754,12 @@ int pq3d_hmat_time_mvm(void* hm, int reps, int fix_tn, float* ms_per) {
     hipEvent_t a, b;
     PQ_HIP(hipEventCreate(&a));
     PQ_HIP(hipEventCreate(&b));
-    hmat_launch(h, h->xin, h->xin + n, nullptr, nullptr, 0.0, fix_tn, h->yout, 0);
+    Launch L;
+    hmat_launch(h, h->xin, h->xin + n, nullptr, nullptr, 0.0, fix_tn, h->yout, L);
     PQ_HIP(hipEventRecord(a, 0));
-    for (int r = 0; r < reps; ++r) hmat_launch(h, h->xin, h->xin + n, nullptr, nullptr, 0.0, fix_tn, h->yout, 0);
+    for (int r = 0; r < reps; ++r) hmat_launch(h, h->xin, h->xin + n, nullptr, nullptr, 0.0, fix_tn, h->yout, L);
     PQ_HIP(hipEventRecord(b, 0));
+    PQ_HIP(L.err);
     PQ_HIP(hipEventSynchronize(b));
     float ms = 0;
     PQ_HIP(hipEventElapsedTime(&ms, a, b));

@@ -666,7 +666,7 @@ __global__ __launch_bounds__(256) void dila_kernel(double* __restrict__ F, long 
 
 // H-matrix traction rates (hmat_mvm.hip): out = 3 x n element-order sums
 void hmat_launch(void* hm, const double* V1, const double* V2, const double* c0, const double* s0, double Vpl,
-                 int fix_tn, double* out, hipStream_t st);
+                 int fix_tn, double* out, Launch& L);
 long long hmat_n(void* hm);
 
 struct DenseSim {
@@ -723,7 +723,7 @@ static SimArgs make_args(DenseSim* s, double* records, long long rec_index) {
 }
 
 template <class T>
-static void launch_matvec_t(DenseSim* s, hipStream_t st) {
+static void launch_matvec_t(DenseSim* s, Launch& L) {
     MatPtrs<T> M;
     for (int k = 0; k < 6; ++k) M.m[k] = (const T*)s->mats[k];
     const int VN = Vec16<T>::n;
@@ -732,34 +732,33 @@ static void launch_matvec_t(DenseSim* s, hipStream_t st) {
     const long long n = s->n;
     // with Bs = Bd = 0 the normal partials are exactly +0 either way
     if (s->fix_tn || s->skip_normal)
-        hipLaunchKernelGGL((matvec_kernel<T, true>), grid, dim3(256), 0, st, M, s->ld, n, s->kc, F + F_SLIPV1 * n,
-                           F + F_SLIPV2 * n, F + F_COSR0 * n, F + F_SINR0 * n, s->Vpl, s->partial);
+        L(matvec_kernel<T, true>, grid, dim3(256), M, s->ld, n, s->kc, F + F_SLIPV1 * n, F + F_SLIPV2 * n,
+          F + F_COSR0 * n, F + F_SINR0 * n, s->Vpl, s->partial);
     else
-        hipLaunchKernelGGL((matvec_kernel<T, false>), grid, dim3(256), 0, st, M, s->ld, n, s->kc,
-                           F + F_SLIPV1 * n, F + F_SLIPV2 * n, F + F_COSR0 * n, F + F_SINR0 * n, s->Vpl,
-                           s->partial);
+        L(matvec_kernel<T, false>, grid, dim3(256), M, s->ld, n, s->kc, F + F_SLIPV1 * n, F + F_SLIPV2 * n,
+          F + F_COSR0 * n, F + F_SINR0 * n, s->Vpl, s->partial);
 }
 
-static void launch_matvec(DenseSim* s, hipStream_t st) {
+static void launch_matvec(DenseSim* s, Launch& L) {
     if (s->hmat) {
         double* F = s->fields;
         const long long n = s->n;
         hmat_launch(s->hmat, F + F_SLIPV1 * n, F + F_SLIPV2 * n, F + F_COSR0 * n, F + F_SINR0 * n, s->Vpl,
-                    s->fix_tn, s->partial, st);
+                    s->fix_tn, s->partial, L);
         return;
     }
-    if (s->mat_type == MAT_F64) launch_matvec_t<double>(s, st);
-    else if (s->mat_type == MAT_F32) launch_matvec_t<float>(s, st);
-    else launch_matvec_t<unsigned short>(s, st);
+    if (s->mat_type == MAT_F64) launch_matvec_t<double>(s, L);
+    else if (s->mat_type == MAT_F32) launch_matvec_t<float>(s, L);
+    else launch_matvec_t<unsigned short>(s, L);
 }
 
-static void launch_step(DenseSim* s, hipStream_t st, double* records, long long rec_index) {
-    launch_matvec(s, st);
+static void launch_step(DenseSim* s, Launch& L, double* records, long long rec_index) {
+    launch_matvec(s, L);
     SimArgs a = make_args(s, records, rec_index);
-    hipLaunchKernelGGL(rk_step_kernel, dim3((unsigned)s->nblocks), dim3(256), 0, st, a);
+    L(rk_step_kernel, dim3((unsigned)s->nblocks), dim3(256), a);
     if (s->dila_on)
-        hipLaunchKernelGGL(dila_kernel, dim3((unsigned)((s->n + 255) / 256)), dim3(256), 0, st, s->fields, s->n,
-                           (const Ctrl*)s->ctrl, s->dila_parr, (const double*)s->dila_tri, s->dila, -1.0);
+        L(dila_kernel, dim3((unsigned)((s->n + 255) / 256)), dim3(256), s->fields, s->n, (const Ctrl*)s->ctrl,
+          s->dila_parr, (const double*)s->dila_tri, s->dila, -1.0);
 }
 
 static int configure(DenseSim* s) {
@@ -1166,21 +1165,21 @@ int pq3d_dense_get_structure(void* h, int* mask, int* skip_normal) {
 
 // The traction product as the step consumes it: split-K partials, then their
 // fixed-order sum into the traction-rate fields.
-static void launch_product(DenseSim* s, hipStream_t st) {
-    launch_matvec(s, st);
+static void launch_product(DenseSim* s, Launch& L) {
+    launch_matvec(s, L);
     const long long n = s->n;
     double* F = s->fields;
-    hipLaunchKernelGGL(reduce_partials_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, s->partial,
-                       s->nchunk, n, F + F_NORMAL_LOADING * n, s->fix_tn, F + F_ADOTV1 * n, F + F_ADOTV2 * n,
-                       F + F_DSIGMADT * n);
+    L(reduce_partials_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), s->partial, s->nchunk, n,
+      F + F_NORMAL_LOADING * n, s->fix_tn, F + F_ADOTV1 * n, F + F_ADOTV2 * n, F + F_DSIGMADT * n);
 }
 
 int pq3d_dense_matvec(void* h) {
     clear_error();
     DenseSim* s = (DenseSim*)h;
     PQ_CHECK(s->mat_type != MAT_NONE || s->hmat, "dense sim: no matrices and no H-matrix attached");
-    launch_product(s, 0);
-    PQ_HIP(hipGetLastError());
+    Launch L;
+    launch_product(s, L);
+    PQ_HIP(L.err);
     PQ_HIP(hipStreamSynchronize(0));
     return 0;
 }
@@ -1233,10 +1232,12 @@ int pq3d_dense_time_product(void* h, int reps, float* ms_per) {
     hipEvent_t a, b;
     PQ_HIP(hipEventCreate(&a));
     PQ_HIP(hipEventCreate(&b));
-    launch_product(s, 0);
+    Launch L;
+    launch_product(s, L);
     PQ_HIP(hipEventRecord(a, 0));
-    for (int r = 0; r < reps; ++r) launch_product(s, 0);
+    for (int r = 0; r < reps; ++r) launch_product(s, L);
     PQ_HIP(hipEventRecord(b, 0));
+    PQ_HIP(L.err);
     PQ_HIP(hipEventSynchronize(b));
     float ms = 0;
     PQ_HIP(hipEventElapsedTime(&ms, a, b));
@@ -1254,10 +1255,12 @@ int pq3d_dense_time_matvec(void* h, int reps, float* ms_per) {
     hipEvent_t a, b;
     PQ_HIP(hipEventCreate(&a));
     PQ_HIP(hipEventCreate(&b));
-    launch_matvec(s, 0);
+    Launch L;
+    launch_matvec(s, L);
     PQ_HIP(hipEventRecord(a, 0));
-    for (int r = 0; r < reps; ++r) launch_matvec(s, 0);
+    for (int r = 0; r < reps; ++r) launch_matvec(s, L);
     PQ_HIP(hipEventRecord(b, 0));
+    PQ_HIP(L.err);
     PQ_HIP(hipEventSynchronize(b));
     float ms = 0;
     PQ_HIP(hipEventElapsedTime(&ms, a, b));
@@ -1274,8 +1277,9 @@ int pq3d_dense_step(void* h, double dttry, double* h_out, double* dtnext_out, in
     DenseSim* s = (DenseSim*)h;
     PQ_CHECK(s->mat_type != MAT_NONE || s->hmat, "dense sim: no matrices and no H-matrix attached");
     PQ_HIP(hipMemcpy(&s->ctrl->dttry, &dttry, sizeof(double), hipMemcpyHostToDevice));
-    launch_step(s, 0, nullptr, -1);
-    PQ_HIP(hipGetLastError());
+    Launch L;
+    launch_step(s, L, nullptr, -1);
+    PQ_HIP(L.err);
     PQ_HIP(hipStreamSynchronize(0));
     Ctrl c;
     PQ_HIP(hipMemcpy(&c, s->ctrl, sizeof(Ctrl), hipMemcpyDeviceToHost));
@@ -1320,21 +1324,22 @@ int pq3d_dense_run(void* h, long long nsteps, int graph_steps, double* records, 
     if (!records) {
         if (!s->graph || s->graph_steps != graph_steps) {
             if (s->graph) (void)hipGraphExecDestroy(s->graph);
-            hipStream_t cs;
-            PQ_HIP(hipStreamCreateWithFlags(&cs, hipStreamNonBlocking));
-            PQ_HIP(hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal));
-            for (int k = 0; k < graph_steps; ++k) launch_step(s, cs, nullptr, -1);
+            s->graph = nullptr;
             hipGraph_t g;
-            PQ_HIP(hipStreamEndCapture(cs, &g));
-            PQ_HIP(hipGraphInstantiate(&s->graph, g, nullptr, nullptr, 0));
+            PQ_HIP(hipGraphCreate(&g, 0));
+            Launch G(st, g);
+            for (int k = 0; k < graph_steps; ++k) launch_step(s, G, nullptr, -1);
+            hipError_t e = G.err;
+            if (e == hipSuccess) e = hipGraphInstantiate(&s->graph, g, nullptr, nullptr, 0);
             (void)hipGraphDestroy(g);
-            (void)hipStreamDestroy(cs);
+            PQ_HIP(e);
             s->graph_steps = graph_steps;
         }
         for (; done + graph_steps <= nsteps; done += graph_steps) PQ_HIP(hipGraphLaunch(s->graph, st));
     }
-    for (; done < nsteps; ++done) launch_step(s, st, records ? s->d_records : nullptr, records ? done : -1);
-    PQ_HIP(hipGetLastError());
+    Launch L(st);
+    for (; done < nsteps; ++done) launch_step(s, L, records ? s->d_records : nullptr, records ? done : -1);
+    PQ_HIP(L.err);
     PQ_HIP(hipStreamSynchronize(0));
     Ctrl c;
     PQ_HIP(hipMemcpy(&c, s->ctrl, sizeof(Ctrl), hipMemcpyDeviceToHost));

@@ -6,6 +6,8 @@
   all complete and the resident footprint (meshes resident or loading) never
   exceeds the budget; the budget formula bounds the memory the library
   reports for a resident mesh.
+* Step graphs built and launched on one thread while another thread loads a
+  mesh (allocations, copies, synchronous calls) come out whole.
 """
 import numpy as np
 import pytest
@@ -77,3 +79,39 @@ def test_admission_under_hbm_budget():
     srv.close()
     assert m["requests"] == 8 and m["failed"] == 0
     assert peak[0] <= budget
+
+
+def test_graphs_built_while_another_thread_loads():
+    import threading
+    from pyquake3d_amd.serving import _Mesh
+    meshes, _ = _meshes()
+    mu = 32038120320.0
+    a = _Mesh("a", *meshes["a"], mu, mu, True)
+    a.load(tune=False)
+    stop, errors, loads = threading.Event(), [], [0]
+
+    def loader():
+        try:
+            while not stop.is_set():
+                b = _Mesh("b", *meshes["b"], mu, mu, True)
+                b.load(tune=False)
+                b.unload()
+                loads[0] += 1
+        except Exception as e:  # reported below
+            errors.append(e)
+
+    t = threading.Thread(target=loader)
+    t.start()
+    k = 0
+    try:
+        while loads[0] < 3 and not errors:   # graphs built all through three loads
+            g = 1 + k % 3            # a different graph length each call: a new graph every time
+            a.ens.run(2 * g, graph_steps=g)
+            k += 1
+            assert k < 100000
+    finally:
+        stop.set()
+        t.join()
+    a.unload()
+    assert not errors, errors
+    print(f"{k} graphs built during {loads[0]} loads", flush=True)
