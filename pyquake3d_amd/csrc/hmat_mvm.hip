@@ -9,19 +9,24 @@
 //   * dense leaf item: FP32 [matrix 6][column n][row msp], msp = ms rounded
 //     up to 4 (zero rows), so a lane loads 4 rows of a column as one 16-byte
 //     vector;
-//   * low-rank item: per matrix the segment's rows of U, [rank k][row mpu] in
-//     the factor type (FP32 or BF16), mpu = ms rounded up to one 16-byte
-//     vector of rows.
-// Low-rank V factors are stored per (block, matrix) as slabs of VEC rows,
-// [slab][column][VEC] (VEC = 16 bytes of the factor type), so a lane loads
-// one 16-byte vector per column.
+//   * low-rank item: the segment's rows of U, [column][row mpu] in the factor
+//     type (FP32 or BF16), mpu = ms rounded up to one 16-byte vector of rows;
+//     the columns grouped by output (A1s then A1d, A2s then A2d, Bs then Bd),
+//     so each output's columns are one contiguous run.
+// Low-rank V factors: per block, the rank rows of the three matrices applied
+// to x1 (A1s, A2s, Bs) one after another, padded to a multiple of VEC rows,
+// then those applied to x2 (A1d, A2d, Bd) likewise; stored as slabs of VEC
+// rows, [slab][column][VEC] (VEC = 16 bytes of the factor type), so a lane
+// loads one 16-byte vector per column.  t (= V x) follows the same row order.
 //
 // Product y = (A1s x1 + A1d x2, A2s x1 + A2d x2, Bs x1 + Bd x2), four launches:
 //   1. gather: x in cluster order (FP32);
-//   2. phase A: t = V x per (block, matrix, slab) -- one wave, FP32 sums;
-//   3. phase B: per wave (one dense item, or a few low-rank items of a
-//      segment), partial rows = dense * x or sum of U * t -- lanes (row
-//      vector, column group), FP32 sums reduced across groups;
+//   2. phase A + dense: waves t = V x over 1-4 slabs of a block, FP32 sums,
+//      then one wave per dense item, partial rows = dense * x (these do not
+//      need t, so they share the launch);
+//   3. low-rank: per wave a few low-rank items of a segment, partial rows =
+//      sum of U * t -- lanes (row vector, column group), FP32 sums reduced
+//      across groups;
 //   4. reduce: per row, the items of its segment summed in FP64 in a fixed
 //      order (deterministic: results do not depend on scheduling) and
 //      scattered to element order.
@@ -35,17 +40,19 @@
 
 namespace pq3d {
 
+// one phase-A wave: nslab consecutive slabs of a block, ls lanes per slab
 struct AItem {
-    long long v_off;  // first element of the slab (layout [column][VEC])
+    long long v_off;  // first element of the first slab (layout [slab][column][VEC])
     int n, c0;        // columns, first cluster-order column
-    int src;          // 0: x1 (strike matrices), 1: x2 (dip matrices)
-    int t_off;        // first of the VEC outputs in t
-    int comp;         // matrix 0..5
+    int t_off;        // first of the nslab * VEC outputs in t
+    int nslab, ls;    // slabs (1-4), lanes per slab (64 / 2^k >= 16)
+    int src_mask;     // bit s: slab s multiplies x2 (else x1)
+    int fix_mask;     // bit s: slab s holds only Bs / Bd rows (or padding): skipped with fix_tn
     int pad;
 };
 
 struct BItem {
-    long long off[6];  // dense: off[0] = data [6][n][ms]; low-rank: U rows [k][ms] per matrix
+    long long off[6];  // dense: off[0] = data [n][6][msp]; low-rank: off[o] = first U column of output o
     int t_off[6];      // low-rank: t offsets per matrix
     int k[6];          // low-rank ranks
     int c0, n;         // dense: first cluster-order column, column count
@@ -63,8 +70,9 @@ struct BWave {
 };
 
 struct Seg {
-    int s0, ms, wave0, wave1;  // the segment's waves; their slots are consecutive
-    int slot0, pad;
+    int s0, ms;  // rows [s0, s0+ms)
+    int nw;      // the segment's phase-B waves; their partial slots are consecutive from slot0
+    int slot0;
 };
 
 struct HMat {
@@ -77,8 +85,8 @@ struct HMat {
     long long na = 0;
     BItem* b_items = nullptr;
     long long nbi = 0;
-    BWave* b_waves = nullptr;
-    long long nbw = 0;
+    BWave* b_waves = nullptr;  // the dense waves, then the low-rank waves
+    long long nbw = 0, nbd = 0;
     Seg* segs = nullptr;
     long long nseg = 0;
     float* ddata = nullptr;   // dense leaves
@@ -133,6 +141,10 @@ template <> struct Slab<float> {
         o[3] = a.w;
     }
     __device__ static float one(const float* p, long long i) { return p[i]; }
+    // the 16-byte vector j as loaded, and its element r (r a constant)
+    using Raw = float4;
+    __device__ static Raw raw(const float* base, long long j) { return *reinterpret_cast<const float4*>(base + 4 * j); }
+    __device__ static float at(const Raw& a, int r) { return r == 0 ? a.x : r == 1 ? a.y : r == 2 ? a.z : a.w; }
 };
 template <> struct Slab<unsigned short> {
     static constexpr int VEC = 8;
@@ -146,51 +158,62 @@ template <> struct Slab<unsigned short> {
         }
     }
     __device__ static float one(const unsigned short* p, long long i) { return bf16f(p[i]); }
+    using Raw = uint4;
+    __device__ static Raw raw(const unsigned short* base, long long j) {
+        return *reinterpret_cast<const uint4*>(base + 8 * j);
+    }
+    __device__ static float at(const Raw& a, int r) {
+        const unsigned int w = (r >> 1) == 0 ? a.x : (r >> 1) == 1 ? a.y : (r >> 1) == 2 ? a.z : a.w;
+        return (r & 1) ? bf16f(w >> 16) : bf16f(w & 0xffffu);
+    }
 };
 
-// phase A: one wave per (block, matrix, slab); t[t_off + e] = sum_j V[e][j] x[c0 + j]
+// phase A: lanes [s ls, (s+1) ls) handle slab s of the item, columns
+// strided by ls; t[t_off + s VEC + e] = sum_j V[s][j][e] x[c0 + j], the ls
+// lanes' sums combined in a fixed butterfly.
 template <class T>
-__global__ __launch_bounds__(256) void hmat_phase_a_kernel(const AItem* __restrict__ items, long long na,
-                                                           const T* __restrict__ vdata,
-                                                           const float* __restrict__ xp, long long n,
-                                                           float* __restrict__ t, int fix_tn) {
+__device__ __forceinline__ void phase_a_wave(const AItem& it, const T* __restrict__ vdata,
+                                             const float* __restrict__ xp, long long n, float* __restrict__ t,
+                                             int fix_tn, int lane) {
     constexpr int VEC = Slab<T>::VEC;
-    const long long w = (long long)blockIdx.x * 4 + (threadIdx.x >> 6);
-    const int lane = threadIdx.x & 63;
-    if (w >= na) return;  // whole wave
-    const AItem it = items[w];
-    if (fix_tn && (it.comp == 2 || it.comp == 5)) return;
-    const T* base = vdata + it.v_off;
-    const float* x = xp + (long long)it.src * n + it.c0;
+    const int ls = it.ls;
+    const int sl = lane / ls, cl = lane - sl * ls;
+    const bool live = sl < it.nslab && !(fix_tn && ((it.fix_mask >> sl) & 1));
     float acc[VEC];
 #pragma unroll
     for (int e = 0; e < VEC; ++e) acc[e] = 0.f;
-    int j = lane;
-    for (; j + 192 < it.n; j += 256) {  // four 16-byte loads in flight per lane
-        float v0[VEC], v1[VEC], v2[VEC], v3[VEC];
-        Slab<T>::load(base, j, v0);
-        Slab<T>::load(base, j + 64, v1);
-        Slab<T>::load(base, j + 128, v2);
-        Slab<T>::load(base, j + 192, v3);
-        const float x0 = x[j], x1 = x[j + 64], x2 = x[j + 128], x3 = x[j + 192];
+    if (live) {
+        const T* base = vdata + it.v_off + (long long)sl * it.n * VEC;
+        const float* x = xp + (long long)((it.src_mask >> sl) & 1) * n + it.c0;
+        for (int j0 = cl; j0 < it.n; j0 += 4 * ls) {  // up to four 16-byte loads in flight per lane
+            // past the last column: reload column j0 and weight it by zero
+            // (unconditional loads stay in flight together)
+            typename Slab<T>::Raw v[4];
+            float xs[4];
 #pragma unroll
-        for (int e = 0; e < VEC; ++e)
-            acc[e] = fmaf(v0[e], x0, fmaf(v1[e], x1, fmaf(v2[e], x2, fmaf(v3[e], x3, acc[e]))));
+            for (int u = 0; u < 4; ++u) {
+                const bool live = j0 + u * ls < it.n;
+                const int j = live ? j0 + u * ls : j0;
+                v[u] = Slab<T>::raw(base, j);
+                const float xv = x[j];
+                xs[u] = live ? xv : 0.f;
+            }
+#pragma unroll
+            for (int e = 0; e < VEC; ++e)
+                acc[e] = fmaf(Slab<T>::at(v[0], e), xs[0],
+                              fmaf(Slab<T>::at(v[1], e), xs[1],
+                                   fmaf(Slab<T>::at(v[2], e), xs[2], fmaf(Slab<T>::at(v[3], e), xs[3], acc[e]))));
+        }
     }
-    for (; j < it.n; j += 64) {
-        float v0[VEC];
-        Slab<T>::load(base, j, v0);
-        const float x0 = x[j];
+    for (int o = 1; o < ls; o <<= 1) {
 #pragma unroll
-        for (int e = 0; e < VEC; ++e) acc[e] = fmaf(v0[e], x0, acc[e]);
+        for (int e = 0; e < VEC; ++e) acc[e] += __shfl_xor(acc[e], o, 64);
     }
     float mine = 0.f;
 #pragma unroll
-    for (int e = 0; e < VEC; ++e) {
-        const float s = wave_sum(acc[e]);
-        if (lane == e) mine = s;
-    }
-    if (lane < VEC) t[it.t_off + lane] = mine;
+    for (int e = 0; e < VEC; ++e)
+        if (cl == e) mine = acc[e];
+    if (live && cl < VEC) t[it.t_off + sl * VEC + cl] = mine;
 }
 
 __device__ __forceinline__ int pow2ceil(int x) {
@@ -199,116 +222,22 @@ __device__ __forceinline__ int pow2ceil(int x) {
     return p;
 }
 
-// phase B: one wave per BWave (one dense item, or up to kLowRankPerWave
-// low-rank items of the same segment accumulated together).  Each lane owns R consecutive rows of the
-// segment (one 16-byte vector: 4 FP32 rows of a dense column, or VEC rows of
-// a U column) and a column group g; the groups are summed across lanes at the
-// end in a fixed order.
-template <class T>
-__global__ __launch_bounds__(256) void hmat_phase_b_kernel(const BWave* __restrict__ waves, long long nwaves,
-                                                           const BItem* __restrict__ items,
-                                                           const float* __restrict__ ddata,
-                                                           const T* __restrict__ udata,
-                                                           const float* __restrict__ xp, long long n,
-                                                           const float* __restrict__ t,
-                                                           float* __restrict__ partial, int fix_tn) {
-    constexpr int VR = Slab<T>::VEC;  // U rows per 16-byte vector
-    const long long w = (long long)blockIdx.x * 4 + (threadIdx.x >> 6);
-    const int lane = threadIdx.x & 63;
-    if (w >= nwaves) return;  // whole wave
-    const BWave wv = waves[w];
-    const int ms = items[wv.item0].ms;
-    const int type = items[wv.item0].type;
-    float acc[VR > 4 ? VR : 4][3];
-#pragma unroll
-    for (int r = 0; r < (VR > 4 ? VR : 4); ++r) acc[r][0] = acc[r][1] = acc[r][2] = 0.f;
-    int R, qp, q, g;
-    if (type == 0) {
-        const BItem it = items[wv.item0];
-        R = 4;
-        const int msp = (ms + 3) & ~3, nq = msp >> 2;
-        qp = pow2ceil(nq);
-        const int G = 64 / qp;
-        q = lane & (qp - 1);
-        g = lane / qp;
-        if (q < nq) {
-            const float* D = ddata + it.off[0] + 4 * q;
-            const float* x1 = xp + it.c0;
-            const float* x2 = xp + n + it.c0;
-            const int nm = fix_tn ? 4 : 6;                 // Bs, Bd last (skipped with fix_tn)
-            for (int j0 = g; j0 < it.n; j0 += 2 * G) {     // two columns' loads in flight
-                float4 m[2][6];
-                float a[2], b[2];
-#pragma unroll
-                for (int u = 0; u < 2; ++u) {
-                    const int j = j0 + u * G < it.n ? j0 + u * G : j0;
-                    const float* col = D + (long long)j * 6 * msp;  // [column][matrix][row]
-#pragma unroll
-                    for (int c = 0; c < 6; ++c)
-                        if (c < nm) m[u][c] = *reinterpret_cast<const float4*>(col + c * msp);
-                    const bool live = j0 + u * G < it.n;
-                    a[u] = live ? x1[j] : 0.f;
-                    b[u] = live ? x2[j] : 0.f;
-                }
-#pragma unroll
-                for (int u = 0; u < 2; ++u) {
-                    const float v0[4] = {m[u][0].x, m[u][0].y, m[u][0].z, m[u][0].w};
-                    const float v1[4] = {m[u][1].x, m[u][1].y, m[u][1].z, m[u][1].w};
-                    const float v3[4] = {m[u][2].x, m[u][2].y, m[u][2].z, m[u][2].w};
-                    const float v4[4] = {m[u][3].x, m[u][3].y, m[u][3].z, m[u][3].w};
-#pragma unroll
-                    for (int r = 0; r < 4; ++r) {
-                        acc[r][0] = fmaf(v0[r], a[u], fmaf(v3[r], b[u], acc[r][0]));
-                        acc[r][1] = fmaf(v1[r], a[u], fmaf(v4[r], b[u], acc[r][1]));
-                    }
-                    if (nm == 6) {
-                        const float v2[4] = {m[u][4].x, m[u][4].y, m[u][4].z, m[u][4].w};
-                        const float v5[4] = {m[u][5].x, m[u][5].y, m[u][5].z, m[u][5].w};
-#pragma unroll
-                        for (int r = 0; r < 4; ++r) acc[r][2] = fmaf(v2[r], a[u], fmaf(v5[r], b[u], acc[r][2]));
-                    }
-                }
-            }
-        }
-    } else {
-        R = VR;
-        const int mpu = (ms + VR - 1) / VR * VR, no = mpu / VR;
-        qp = pow2ceil(no);
-        const int G = 64 / qp;
-        q = lane & (qp - 1);
-        g = lane / qp;
-        if (q < no) {
-            for (int x = 0; x < wv.count; ++x) {
-                const BItem& it = items[wv.item0 + x];
-#pragma unroll
-                for (int c = 0; c < 6; ++c) {
-                    if (fix_tn && (c == 2 || c == 5)) continue;
-                    const int k = it.k[c];
-                    const T* U = udata + it.off[c];
-                    const float* tt = t + it.t_off[c];
-                    for (int l = g; l < k; l += G) {
-                        float u[VR];
-                        Slab<T>::load(U + (long long)l * mpu, q, u);
-                        const float tl = tt[l];
-#pragma unroll
-                        for (int r = 0; r < VR; ++r) acc[r][c % 3] = fmaf(u[r], tl, acc[r][c % 3]);
-                    }
-                }
-            }
-        }
-    }
+// Phase-B epilogue: lanes (q, g) hold RA rows q*R.. of column group g; the
+// groups are summed across lanes in a fixed butterfly and group 0 writes the
+// partial rows [3][ms] at P.
+template <int RA>
+__device__ __forceinline__ void b_store(float (&acc)[RA][3], int R, int qp, int q, int g, int ms, float* P) {
     for (int o = qp; o < 64; o <<= 1) {
 #pragma unroll
-        for (int r = 0; r < (VR > 4 ? VR : 4); ++r) {
+        for (int r = 0; r < RA; ++r) {
             acc[r][0] += __shfl_xor(acc[r][0], o, 64);
             acc[r][1] += __shfl_xor(acc[r][1], o, 64);
             acc[r][2] += __shfl_xor(acc[r][2], o, 64);
         }
     }
     if (g == 0) {
-        float* P = partial + wv.slot;
 #pragma unroll
-        for (int r = 0; r < (VR > 4 ? VR : 4); ++r) {
+        for (int r = 0; r < RA; ++r) {
             const int row = q * R + r;
             if (r < R && row < ms) {
                 P[row] = acc[r][0];
@@ -317,6 +246,150 @@ __global__ __launch_bounds__(256) void hmat_phase_b_kernel(const BWave* __restri
             }
         }
     }
+}
+
+// One dense item: each lane owns 4 consecutive rows of the segment (one
+// 16-byte vector of a column) and a column group g.
+__device__ __forceinline__ void dense_wave(const BWave& wv, const BItem* __restrict__ items,
+                                           const float* __restrict__ ddata, const float* __restrict__ xp,
+                                           long long n, float* __restrict__ partial, int fix_tn, int lane) {
+    const BItem it = items[wv.item0];
+    const int ms = it.ms;
+    float acc[4][3];
+#pragma unroll
+    for (int r = 0; r < 4; ++r) acc[r][0] = acc[r][1] = acc[r][2] = 0.f;
+    const int msp = (ms + 3) & ~3, nq = msp >> 2;
+    const int qp = pow2ceil(nq);
+    const int G = 64 / qp;
+    const int q = lane & (qp - 1);
+    const int g = lane / qp;
+    if (q < nq) {
+        const float* D = ddata + it.off[0] + 4 * q;
+        const float* x1 = xp + it.c0;
+        const float* x2 = xp + n + it.c0;
+        const int nm = fix_tn ? 4 : 6;                 // Bs, Bd last (skipped with fix_tn)
+        for (int j0 = g; j0 < it.n; j0 += 2 * G) {     // two columns' loads in flight
+            float4 m[2][6];
+            float a[2], b[2];
+#pragma unroll
+            for (int u = 0; u < 2; ++u) {
+                const int j = j0 + u * G < it.n ? j0 + u * G : j0;
+                const float* col = D + (long long)j * 6 * msp;  // [column][matrix][row]
+#pragma unroll
+                for (int c = 0; c < 6; ++c)
+                    if (c < nm) m[u][c] = *reinterpret_cast<const float4*>(col + c * msp);
+                const bool live = j0 + u * G < it.n;
+                a[u] = live ? x1[j] : 0.f;
+                b[u] = live ? x2[j] : 0.f;
+            }
+#pragma unroll
+            for (int u = 0; u < 2; ++u) {
+                const float v0[4] = {m[u][0].x, m[u][0].y, m[u][0].z, m[u][0].w};
+                const float v1[4] = {m[u][1].x, m[u][1].y, m[u][1].z, m[u][1].w};
+                const float v3[4] = {m[u][2].x, m[u][2].y, m[u][2].z, m[u][2].w};
+                const float v4[4] = {m[u][3].x, m[u][3].y, m[u][3].z, m[u][3].w};
+#pragma unroll
+                for (int r = 0; r < 4; ++r) {
+                    acc[r][0] = fmaf(v0[r], a[u], fmaf(v3[r], b[u], acc[r][0]));
+                    acc[r][1] = fmaf(v1[r], a[u], fmaf(v4[r], b[u], acc[r][1]));
+                }
+                if (nm == 6) {
+                    const float v2[4] = {m[u][4].x, m[u][4].y, m[u][4].z, m[u][4].w};
+                    const float v5[4] = {m[u][5].x, m[u][5].y, m[u][5].z, m[u][5].w};
+#pragma unroll
+                    for (int r = 0; r < 4; ++r) acc[r][2] = fmaf(v2[r], a[u], fmaf(v5[r], b[u], acc[r][2]));
+                }
+            }
+        }
+    }
+    b_store<4>(acc, 4, qp, q, g, ms, partial + wv.slot);
+}
+
+// Launch 2: waves [0, na) are phase-A items, waves [na, na + nd) dense items.
+template <class T>
+__global__ __launch_bounds__(256) void hmat_phase_ad_kernel(const AItem* __restrict__ a_items, long long na,
+                                                            const T* __restrict__ vdata,
+                                                            const BWave* __restrict__ dwaves, long long nd,
+                                                            const BItem* __restrict__ items,
+                                                            const float* __restrict__ ddata,
+                                                            const float* __restrict__ xp, long long n,
+                                                            float* __restrict__ t, float* __restrict__ partial,
+                                                            int fix_tn) {
+    // wave-uniform (readfirstlane): the item tables are then read with scalar loads
+    const long long w = (long long)blockIdx.x * 4 + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
+    if (w < na) {
+        const AItem it = a_items[w];
+        phase_a_wave<T>(it, vdata, xp, n, t, fix_tn, lane);
+    } else if (w - na < nd) {
+        const BWave wv = dwaves[w - na];
+        dense_wave(wv, items, ddata, xp, n, partial, fix_tn, lane);
+    }
+}
+
+// Launch 3: one wave per BWave of up to kLowRankPerWave low-rank items of the
+// same segment accumulated together.  Each lane owns VR consecutive rows of
+// the segment (one 16-byte vector of a U column) and a column group g; output
+// o's columns (those of matrix o, then of matrix o + 3) are taken two per
+// lane at a time, the three outputs' loads issued together.
+template <class T>
+__global__ __launch_bounds__(256) void hmat_lowrank_kernel(const BWave* __restrict__ waves, long long nwaves,
+                                                           const BItem* __restrict__ items,
+                                                           const T* __restrict__ udata,
+                                                           const float* __restrict__ t,
+                                                           float* __restrict__ partial, int fix_tn) {
+    constexpr int VR = Slab<T>::VEC;  // U rows per 16-byte vector
+    // wave-uniform (readfirstlane): the item tables are then read with scalar loads
+    const long long w = (long long)blockIdx.x * 4 + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
+    if (w >= nwaves) return;  // whole wave
+    const BWave wv = waves[w];
+    const int ms = items[wv.item0].ms;
+    float acc[VR][3];
+#pragma unroll
+    for (int r = 0; r < VR; ++r) acc[r][0] = acc[r][1] = acc[r][2] = 0.f;
+    const int mpu = (ms + VR - 1) / VR * VR, no = mpu / VR;
+    const int qp = pow2ceil(no);
+    const int G = 64 / qp;
+    const int q = lane & (qp - 1);
+    const int g = lane / qp;
+    if (q < no) {
+        for (int x = 0; x < wv.count; ++x) {
+            const BItem it = items[wv.item0 + x];  // wave-uniform: held in scalar registers
+            int K[3], mid[3];
+#pragma unroll
+            for (int o = 0; o < 3; ++o) {
+                mid[o] = it.k[o];
+                K[o] = (fix_tn && o == 2) ? 0 : it.k[o] + it.k[o + 3];
+            }
+            const int kmax = max(K[0], max(K[1], K[2]));
+            for (int l0 = g; l0 < kmax; l0 += 2 * G) {
+                typename Slab<T>::Raw u[3][2];
+                float tl[3][2];
+#pragma unroll
+                for (int o = 0; o < 3; ++o)
+#pragma unroll
+                    for (int v = 0; v < 2; ++v) {
+                        // past the output's last column: reload the item's
+                        // first column and weight it by zero (unconditional
+                        // loads stay in flight together)
+                        const int l = l0 + v * G;
+                        const bool live = l < K[o];
+                        const long long col = live ? it.off[o] + (long long)l * mpu : it.off[0];
+                        u[o][v] = Slab<T>::raw(udata + col, q);
+                        const float tv = t[!live ? it.t_off[0] : l < mid[o] ? it.t_off[o] + l : it.t_off[o + 3] + l - mid[o]];
+                        tl[o][v] = live ? tv : 0.f;
+                    }
+#pragma unroll
+                for (int o = 0; o < 3; ++o)
+#pragma unroll
+                    for (int r = 0; r < VR; ++r)
+                        acc[r][o] = fmaf(Slab<T>::at(u[o][0], r), tl[o][0],
+                                         fmaf(Slab<T>::at(u[o][1], r), tl[o][1], acc[r][o]));
+            }
+        }
+    }
+    b_store<VR>(acc, VR, qp, q, g, ms, partial + wv.slot);
 }
 
 // reduce: per cluster-order row, eight lanes each sum every eighth item of
@@ -338,7 +411,7 @@ __global__ __launch_bounds__(256) void hmat_reduce_kernel(const Seg* __restrict_
         const int i = (int)(p - s.s0);
         const long long stride = 3LL * s.ms;
         const float* P = partial + s.slot0 + i + stride * sub;
-        for (int it = s.wave0 + sub; it < s.wave1; it += 8, P += 8 * stride) {
+        for (int it = sub; it < s.nw; it += 8, P += 8 * stride) {
             y0 += (double)P[0];
             y1 += (double)P[s.ms];
             y2 += (double)P[2 * s.ms];
@@ -375,12 +448,12 @@ static void launch_mvm_t(HMat* h, const double* V1, const double* V2, const doub
                          double Vpl, int fix_tn, double* out, Launch& L) {
     const long long n = h->n;
     L(hmat_gather_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), h->perm, n, V1, V2, c0, s0, Vpl, h->xp);
-    if (h->na)
-        L(hmat_phase_a_kernel<T>, dim3((unsigned)((h->na + 3) / 4)), dim3(256), h->a_items, h->na,
-          (const T*)h->vdata, h->xp, n, h->t, fix_tn);
-    if (h->nbw)
-        L(hmat_phase_b_kernel<T>, dim3((unsigned)((h->nbw + 3) / 4)), dim3(256), h->b_waves, h->nbw, h->b_items,
-          h->ddata, (const T*)h->udata, h->xp, n, h->t, h->partial, fix_tn);
+    if (h->na + h->nbd)
+        L(hmat_phase_ad_kernel<T>, dim3((unsigned)((h->na + h->nbd + 3) / 4)), dim3(256), h->a_items, h->na,
+          (const T*)h->vdata, h->b_waves, h->nbd, h->b_items, h->ddata, h->xp, n, h->t, h->partial, fix_tn);
+    if (h->nbw > h->nbd)
+        L(hmat_lowrank_kernel<T>, dim3((unsigned)((h->nbw - h->nbd + 3) / 4)), dim3(256), h->b_waves + h->nbd,
+          h->nbw - h->nbd, h->b_items, (const T*)h->udata, h->t, h->partial, fix_tn);
     L(hmat_reduce_kernel, dim3((unsigned)((8 * n + 255) / 256)), dim3(256), h->segs, h->row_seg, h->perm, n,
       h->partial, out);
 }
@@ -448,7 +521,7 @@ static int pack(HMat* h, long long n, const int* perm, long long nb, const long 
         const int pieces = (len + 63) / 64;
         for (int q = 0; q < pieces; ++q) {
             const int s = a + (int)((long long)len * q / pieces), e = a + (int)((long long)len * (q + 1) / pieces);
-            segs.push_back(Seg{s, e - s, 0, 0, 0, 0});
+            segs.push_back(Seg{s, e - s, 0, 0});
         }
     }
     std::vector<int> seg_start(segs.size());
@@ -492,7 +565,10 @@ static int pack(HMat* h, long long n, const int* perm, long long nb, const long 
         for (long long j = col_off[b]; j < col_off[b + 1]; ++j)
             cinv[cinv_off[b] + inv[col_idx[j]] - c0[b]] = (int)(j - col_off[b]);
     }
-    // t offsets and V slabs (phase A)
+    // t offsets and V slabs (phase A): per block the rank rows of A1s, A2s, Bs
+    // (applied to x1) padded to a multiple of VEC, then those of A1d, A2d, Bd
+    // (applied to x2) likewise; phase-A waves take up to 64 / ls consecutive
+    // slabs of a block, ls lanes per slab (about four columns per lane)
     std::vector<int> toff(nb * 6, 0);
     std::vector<AItem> aitems;
     std::vector<T> vpack;
@@ -501,41 +577,73 @@ static int pack(HMat* h, long long n, const int* perm, long long nb, const long 
     for (long long b = 0; b < nb; ++b) {
         if (kind[b] == 0) continue;
         ++h->lowrank_blocks;
-        const long long m = row_off[b + 1] - row_off[b], nc = col_off[b + 1] - col_off[b];
-        for (int c = 0; c < 6; ++c) {
-            const long long k = ranks[6 * b + c];
-            toff[6 * b + c] = (int)tcount;
-            if (k == 0) continue;
-            const long long kp = (k + VEC - 1) / VEC * VEC;
-            const double* Vb = V + voff[6 * b + c];
-            const long long base = (long long)vpack.size();
-            vpack.resize((size_t)(base + kp * nc));
-            for (long long lb = 0; lb < kp / VEC; ++lb) {
-                for (long long jp = 0; jp < nc; ++jp) {
-                    const int j = cinv[cinv_off[b] + jp];
-                    for (int e = 0; e < VEC; ++e) {
-                        const long long l = lb * VEC + e;
-                        vpack[(size_t)(base + (lb * nc + jp) * VEC + e)] = conv<T>(l < k ? Vb[l * nc + j] : 0.0);
+        const long long nc = col_off[b + 1] - col_off[b];
+        int ls = 16;
+        while (ls < 64 && 4LL * ls < nc) ls <<= 1;
+        const int S = 64 / ls;
+        std::vector<int> slab_src, slab_fix;
+        const long long base = (long long)vpack.size();
+        for (int gi = 0; gi < 2; ++gi) {
+            const long long* kr = ranks + 6 * b + 3 * gi;
+            const long long rows = kr[0] + kr[1] + kr[2], nsl = (rows + VEC - 1) / VEC;
+            const long long row0 = (long long)slab_src.size() * VEC;  // first t row of the group
+            long long start = 0;
+            for (int c = 3 * gi; c < 3 * gi + 3; ++c) {
+                toff[6 * b + c] = (int)(tcount + row0 + start);
+                start += ranks[6 * b + c];
+            }
+            const long long s0 = base + row0 * nc;
+            vpack.resize((size_t)(s0 + nsl * VEC * nc), T(0));
+            long long r = 0;
+            for (int c = 3 * gi; c < 3 * gi + 3; ++c) {
+                const double* Vb = V + voff[6 * b + c];
+                for (long long l = 0; l < ranks[6 * b + c]; ++l, ++r) {
+                    const long long sl = r / VEC, e = r % VEC;
+                    for (long long jp = 0; jp < nc; ++jp) {
+                        const int j = cinv[cinv_off[b] + jp];
+                        vpack[(size_t)(s0 + (sl * nc + jp) * VEC + e)] = conv<T>(Vb[l * nc + j]);
                     }
                 }
-                aitems.push_back(AItem{base + lb * nc * VEC, (int)nc, c0[b], c / 3, (int)(tcount + lb * VEC), c, 0});
+                const double ab = (double)ranks[6 * b + c] * nc * sizeof(T);
+                h->alg += ab;
+                h->alg_factors += ab;
+                if (c != 2 && c != 5) h->alg_fix += ab;
             }
-            const double ab = (double)k * nc * sizeof(T), sb = (double)kp * nc * sizeof(T);
-            h->alg += ab;
-            h->stored += sb;
-            h->alg_factors += ab;
-            if (c != 2 && c != 5) {
-                h->alg_fix += ab;
-                h->stored_fix += sb;
+            const long long keep = kr[0] + kr[1];  // rows before Bs / Bd
+            for (long long sl = 0; sl < nsl; ++sl) {
+                slab_src.push_back(gi);
+                slab_fix.push_back(sl * VEC >= keep ? 1 : 0);
             }
-            tcount += kp;
         }
-        (void)m;
+        const long long nsl = (long long)slab_src.size();
+        for (long long s0 = 0; s0 < nsl; s0 += S) {
+            AItem a;
+            memset(&a, 0, sizeof(a));
+            a.v_off = base + s0 * VEC * nc;
+            a.n = (int)nc;
+            a.c0 = c0[b];
+            a.t_off = (int)(tcount + s0 * VEC);
+            a.nslab = (int)std::min<long long>(S, nsl - s0);
+            a.ls = ls;
+            for (int q = 0; q < a.nslab; ++q) {
+                a.src_mask |= slab_src[s0 + q] << q;
+                a.fix_mask |= slab_fix[s0 + q] << q;
+            }
+            aitems.push_back(a);
+        }
+        double sb = 0, sbf = 0;
+        for (long long sl = 0; sl < nsl; ++sl) {
+            sb += (double)VEC * nc * sizeof(T);
+            if (!slab_fix[sl]) sbf += (double)VEC * nc * sizeof(T);
+        }
+        h->stored += sb;
+        h->stored_fix += sbf;
+        tcount += nsl * VEC;
     }
     // phase B items: per segment the dense leaves first, then the low-rank
     // blocks, in block order within each kind
     std::vector<BItem> bitems;
-    std::vector<BWave> bwaves;
+    std::vector<BWave> dwaves, lwaves;
     std::vector<float> dpack;
     std::vector<T> upack;
     long long slot = 0;
@@ -582,29 +690,35 @@ static int pack(HMat* h, long long n, const int* perm, long long nb, const long 
                 h->stored_fix += sb * 4 / 6;
                 h->alg_dense += ab;
             } else {
-                // U columns padded to a multiple of VEC rows (16-byte vectors)
+                // U columns padded to a multiple of VEC rows (16-byte vectors);
+                // per output o the columns of matrix o, then of matrix o + 3
                 const int mpu = (ms + VEC - 1) / VEC * VEC;
                 for (int c = 0; c < 6; ++c) {
-                    const long long k = ranks[6 * b + c];
-                    it.k[c] = (int)k;
+                    it.k[c] = (int)ranks[6 * b + c];
                     it.t_off[c] = toff[6 * b + c];
+                }
+                for (int o = 0; o < 3; ++o) {
                     const long long base = (long long)upack.size();
-                    it.off[c] = base;
-                    if (k == 0) continue;
-                    upack.resize((size_t)(base + k * mpu), T(0));
-                    const double* Ub = U + uoff[6 * b + c];
-                    for (long long l = 0; l < k; ++l)
-                        for (int ip = 0; ip < ms; ++ip) {
-                            const int i = rinv[rinv_off[b] + lr0 + ip];
-                            upack[(size_t)(base + l * mpu + ip)] = conv<T>(Ub[l * m + i]);
+                    it.off[o] = base;
+                    const long long kk = ranks[6 * b + o] + ranks[6 * b + o + 3];
+                    upack.resize((size_t)(base + kk * mpu), T(0));
+                    long long col = 0;
+                    for (int c = o; c < 6; c += 3) {
+                        const long long k = ranks[6 * b + c];
+                        const double* Ub = U + uoff[6 * b + c];
+                        for (long long l = 0; l < k; ++l, ++col)
+                            for (int ip = 0; ip < ms; ++ip) {
+                                const int i = rinv[rinv_off[b] + lr0 + ip];
+                                upack[(size_t)(base + col * mpu + ip)] = conv<T>(Ub[l * m + i]);
+                            }
+                        const double ab = (double)k * ms * sizeof(T), sb = (double)k * mpu * sizeof(T);
+                        h->alg += ab;
+                        h->stored += sb;
+                        h->alg_factors += ab;
+                        if (c != 2 && c != 5) {
+                            h->alg_fix += ab;
+                            h->stored_fix += sb;
                         }
-                    const double ab = (double)k * ms * sizeof(T), sb = (double)k * mpu * sizeof(T);
-                    h->alg += ab;
-                    h->stored += sb;
-                    h->alg_factors += ab;
-                    if (c != 2 && c != 5) {
-                        h->alg_fix += ab;
-                        h->stored_fix += sb;
                     }
                 }
             }
@@ -613,20 +727,22 @@ static int pack(HMat* h, long long n, const int* perm, long long nb, const long 
         // waves: one per dense item, kLowRankPerWave low-rank items per wave
         // (many small waves hide memory latency better: 8 per wave measured
         // 15% slower than 1 on the 60k product)
-        sg.wave0 = (int)bwaves.size();
         sg.slot0 = (int)slot;
         const int last_item = (int)bitems.size();
         for (int i = first_item; i < last_item;) {
             int cnt = 1;
             if (bitems[i].type == 1)
                 while (cnt < kLowRankPerWave && i + cnt < last_item && bitems[i + cnt].type == 1) ++cnt;
-            bwaves.push_back(BWave{i, cnt, (int)slot, 0});
+            (bitems[i].type == 0 ? dwaves : lwaves).push_back(BWave{i, cnt, (int)slot, 0});
             for (int q = 0; q < cnt; ++q) bitems[i + q].slot = (int)slot;
             slot += 3 * sg.ms;
+            ++sg.nw;
             i += cnt;
         }
-        sg.wave1 = (int)bwaves.size();
     }
+    // dense waves first (launch 2), then the low-rank waves (launch 3)
+    std::vector<BWave> bwaves(dwaves);
+    bwaves.insert(bwaves.end(), lwaves.begin(), lwaves.end());
     PQ_CHECK(slot < INT32_MAX && tcount < INT32_MAX, "pq3d_hmat_create: H-matrix too large for 32-bit offsets");
     std::vector<int> row_seg(n);
     for (size_t s = 0; s < segs.size(); ++s)
@@ -636,6 +752,7 @@ static int pack(HMat* h, long long n, const int* perm, long long nb, const long 
     h->na = (long long)aitems.size();
     h->nbi = (long long)bitems.size();
     h->nbw = (long long)bwaves.size();
+    h->nbd = (long long)dwaves.size();
     h->nseg = (long long)segs.size();
     h->nt = tcount;
     h->npart = slot;
