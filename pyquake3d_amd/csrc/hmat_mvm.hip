@@ -144,6 +144,8 @@ template <> struct Slab<float> {
     // the 16-byte vector j as loaded, and its element r (r a constant)
     using Raw = float4;
     __device__ static Raw raw(const float* base, long long j) { return *reinterpret_cast<const float4*>(base + 4 * j); }
+    // the vector at element offset e (32-bit) from a wave-uniform base
+    __device__ static Raw raw32(const float* base, unsigned e) { return *reinterpret_cast<const float4*>(base + e); }
     __device__ static float at(const Raw& a, int r) { return r == 0 ? a.x : r == 1 ? a.y : r == 2 ? a.z : a.w; }
 };
 template <> struct Slab<unsigned short> {
@@ -161,6 +163,9 @@ template <> struct Slab<unsigned short> {
     using Raw = uint4;
     __device__ static Raw raw(const unsigned short* base, long long j) {
         return *reinterpret_cast<const uint4*>(base + 8 * j);
+    }
+    __device__ static Raw raw32(const unsigned short* base, unsigned e) {
+        return *reinterpret_cast<const uint4*>(base + e);
     }
     __device__ static float at(const Raw& a, int r) {
         const unsigned int w = (r >> 1) == 0 ? a.x : (r >> 1) == 1 ? a.y : (r >> 1) == 2 ? a.z : a.w;
@@ -183,8 +188,11 @@ __device__ __forceinline__ void phase_a_wave(const AItem& it, const T* __restric
 #pragma unroll
     for (int e = 0; e < VEC; ++e) acc[e] = 0.f;
     if (live) {
-        const T* base = vdata + it.v_off + (long long)sl * it.n * VEC;
-        const float* x = xp + (long long)((it.src_mask >> sl) & 1) * n + it.c0;
+        // 32-bit element offsets from wave-uniform bases
+        const T* base = vdata + it.v_off;
+        const float* x = xp + it.c0;
+        const unsigned vs = (unsigned)(sl * it.n * VEC);
+        const unsigned xs0 = ((it.src_mask >> sl) & 1) ? (unsigned)n : 0u;
         for (int j0 = cl; j0 < it.n; j0 += 4 * ls) {  // up to four 16-byte loads in flight per lane
             // past the last column: reload column j0 and weight it by zero
             // (unconditional loads stay in flight together)
@@ -193,9 +201,9 @@ __device__ __forceinline__ void phase_a_wave(const AItem& it, const T* __restric
 #pragma unroll
             for (int u = 0; u < 4; ++u) {
                 const bool live = j0 + u * ls < it.n;
-                const int j = live ? j0 + u * ls : j0;
-                v[u] = Slab<T>::raw(base, j);
-                const float xv = x[j];
+                const unsigned j = (unsigned)(live ? j0 + u * ls : j0);
+                v[u] = Slab<T>::raw32(base, vs + j * VEC);
+                const float xv = x[xs0 + j];
                 xs[u] = live ? xv : 0.f;
             }
 #pragma unroll
@@ -356,12 +364,20 @@ __global__ __launch_bounds__(256) void hmat_lowrank_kernel(const BWave* __restri
     if (q < no) {
         for (int x = 0; x < wv.count; ++x) {
             const BItem it = items[wv.item0 + x];  // wave-uniform: held in scalar registers
-            int K[3], mid[3];
+            // per output: columns K, of which the first mid are matrix o's;
+            // U element offsets from the item's first column, t indices
+            // ta + l (l < mid) and tb + l
+            int K[3], mid[3], uo[3], ta[3], tb[3];
 #pragma unroll
             for (int o = 0; o < 3; ++o) {
                 mid[o] = it.k[o];
                 K[o] = (fix_tn && o == 2) ? 0 : it.k[o] + it.k[o + 3];
+                uo[o] = (int)(it.off[o] - it.off[0]);
+                ta[o] = it.t_off[o];
+                tb[o] = it.t_off[o + 3] - it.k[o];
             }
+            const T* Ub = udata + it.off[0];
+            const unsigned qo = (unsigned)(q * VR);
             const int kmax = max(K[0], max(K[1], K[2]));
             for (int l0 = g; l0 < kmax; l0 += 2 * G) {
                 typename Slab<T>::Raw u[3][2];
@@ -375,9 +391,10 @@ __global__ __launch_bounds__(256) void hmat_lowrank_kernel(const BWave* __restri
                         // loads stay in flight together)
                         const int l = l0 + v * G;
                         const bool live = l < K[o];
-                        const long long col = live ? it.off[o] + (long long)l * mpu : it.off[0];
-                        u[o][v] = Slab<T>::raw(udata + col, q);
-                        const float tv = t[!live ? it.t_off[0] : l < mid[o] ? it.t_off[o] + l : it.t_off[o + 3] + l - mid[o]];
+                        const unsigned e = live ? (unsigned)(uo[o] + l * mpu) : 0u;
+                        u[o][v] = Slab<T>::raw32(Ub, e + qo);
+                        const int ti = l + (l < mid[o] ? ta[o] : tb[o]);
+                        const float tv = t[(unsigned)(live ? ti : ta[0])];
                         tl[o][v] = live ? tv : 0.f;
                     }
 #pragma unroll
