@@ -13,6 +13,9 @@ element size, leaf 32, eta 2, ACA tolerance 1e-4.  Bars:
 * product with FP32 factors: within 1e-5 of sum |U| |V| |x| of the FP64
   block product (oracle restatement of Hmatrix.py:1773-1866); BF16 within
   1e-2; against the exact dense operator, relative 2-norm error <= 10 eps;
+* a planar fault (Bs, Bd of rank 0, exact-zero dense parts), small leaves:
+  the product against the block product as above, its normal output exactly
+  zero;
 * stepping: a simulation whose traction rates come from the H-matrix follows
   the NumPy oracle run on the H-matrix's own dense expansion (FP32-rounded
   factors) to 1e-6 over 30 steps.
@@ -189,6 +192,40 @@ def test_product_matches_block_product(hcase):
         assert np.array_equal(yf[:2], y[:2]) and not yf[2].any()
     info = tree.device().info()
     assert info["bytes_per_mvm"] > 0 and info["segments"] > 0
+
+
+def test_planar_fault_product_with_zero_rank_matrices():
+    """A planar vertical fault: Bs and Bd vanish by symmetry, so the ACA gives
+    them rank 0 in every low-rank block (empty rank groups and V slabs of the
+    dip group only) and the dense leaves hold exact zeros.  Small leaves (16)
+    give short segments and narrow blocks (several V slabs per phase-A wave).
+    The product matches the oracle block product (FP32 factors 1e-5, BF16
+    1e-2 of the magnitude sum), the normal output is exactly zero, and
+    fix_tn changes nothing."""
+    from pyquake3d_amd import Hmatrix as H, _lib, synth
+    nodes_km, tri = synth.planar_fault_mesh(30, 20, 15.0, 10.0)
+    nodes = nodes_km * 1e3
+    tree = H.build(nodes, tri, MU, LAM, True, mini_leaf=16, eps=EPS)
+    low = [b for b in tree.blocks_to_process if b.judaca]
+    assert len(low) > 10
+    assert all(b.factors(name)[0].shape[1] == 0 for b in low for name in ("Bs", "Bd"))
+    n = len(tri)
+    x1, x2 = _rand_x(n, 3)
+    leaves = _leaves(tree)
+    absl = [(r, c, None if d is None else {k: np.abs(v) for k, v in d.items()},
+             None if f is None else {k: (np.abs(U), np.abs(V)) for k, (U, V) in f.items()})
+            for r, c, d, f in leaves]
+    want = [HO.blocks_mvm(x1, leaves, NAMES[q]) + HO.blocks_mvm(x2, leaves, NAMES[q + 3]) for q in range(2)]
+    mag = [HO.blocks_mvm(np.abs(x1), absl, NAMES[q]) + HO.blocks_mvm(np.abs(x2), absl, NAMES[q + 3])
+           for q in range(2)]
+    for ft, tol in ((_lib.MAT_F32, 1e-5), (_lib.MAT_BF16, 1e-2)):
+        dev = tree.device(factor_type=ft)
+        y = dev.mvm(x1, x2)
+        for q in range(2):
+            err = np.abs(y[q] - want[q]) / mag[q]
+            assert err.max() < tol, (ft, q, err.max())
+        assert not y[2].any()
+        assert np.array_equal(dev.mvm(x1, x2, fix_tn=True), y)
 
 
 def test_product_against_exact_operator(hcase):
