@@ -890,11 +890,20 @@ static bool ens_split_active(const Ensemble* e) { return e->prec == 1 && e->kc %
 
 static size_t ens_panel_bytes(const Ensemble* e) { return (size_t)e->npad * (size_t)e->KB * 32 * sizeof(float); }
 
+static void ens_drop_graph(Ensemble* e) {
+    if (e->graph) {
+        (void)hipGraphExecDestroy(e->graph);
+        e->graph = nullptr;
+    }
+}
+
 // Build the panel copies the split GEMM needs (the four shear matrices, and
 // Bs, Bd unless the GEMM leaves them out) from the current matrices and free
-// the others; none at all outside precision mode 1.
+// the others; none at all outside precision mode 1.  Drops the step graph: its
+// nodes hold the panel addresses and the GEMM variant chosen when it was built.
 static int ens_update_panels(Ensemble* e) {
     PQ_HIP(hipStreamSynchronize(e->stream));
+    ens_drop_graph(e);
     e->panels_ready = 0;
     const size_t bytes = ens_panel_bytes(e);
     const long long total4 = (long long)(bytes / sizeof(float4));
@@ -947,13 +956,6 @@ static void ens_launch_step(Ensemble* e, Launch& L) {
     for (int r = 0; r < ENS_ROUNDS; ++r) {
         L(ens_attempt_kernel, dim3(blocks), dim3(256), a, r);
         L(ens_decide_kernel, dim3(blocks), dim3(256), a, r);
-    }
-}
-
-static void ens_drop_graph(Ensemble* e) {
-    if (e->graph) {
-        (void)hipGraphExecDestroy(e->graph);
-        e->graph = nullptr;
     }
 }
 
@@ -1276,7 +1278,6 @@ int pq3d_ens_get_precision(void* h, int* mode, int* split_active) {
 // (the structure is rescanned whenever the matrices change).
 static int ens_set_structure(Ensemble* e, int mask) {
     const int skip = (mask & 0x24) == 0x24;
-    if (skip != e->skip_normal) ens_drop_graph(e);
     e->zero_mask = mask;
     e->skip_normal = skip;
     return ens_update_panels(e);
