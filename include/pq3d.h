@@ -71,6 +71,17 @@ int pq3d_event_destroy(void* ev);
 int pq3d_event_record(void* ev);
 int pq3d_event_elapsed_ms(void* start, void* stop, float* ms);
 
+/* Measurement probes (no counterpart in the reference): what this device
+ * sustains for the resources the hot path is bound by, quoted beside the
+ * published peaks.  pq3d_probe_stream: read-only stream over a buffer of
+ * `bytes` it allocates (a contiguous share per workgroup, 16-byte loads, eight
+ * in flight per lane, 32 workgroups per CU), GB/s and ms of the fastest of
+ * `reps` passes.  pq3d_probe_fma64:
+ * independent FP64 FMA chains at full occupancy, TFLOP/s (an FMA = 2 flops)
+ * and ms per launch. */
+int pq3d_probe_stream(long long bytes, int reps, float* ms_per, double* gbs);
+int pq3d_probe_fma64(int iters, int reps, float* ms_per, double* tflops);
+
 /* Matrix storage types */
 #define PQ3D_MAT_F64 0
 #define PQ3D_MAT_F32 1

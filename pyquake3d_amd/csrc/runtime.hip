@@ -88,6 +88,54 @@ int scan_zero_matrices(void* const* mats, int nmat, int mat_type, long long ld, 
 
 using namespace pq3d;
 
+// ---------------------------------------------------------------------------
+// Measurement probes: what this device sustains for the two resources the
+// hot path is bound by, to quote beside the published peaks.
+// ---------------------------------------------------------------------------
+
+// Read-only stream: each workgroup reads one contiguous share of the buffer,
+// its lanes 16-byte vectors 4 KB apart per instruction, eight loads in flight
+// per lane (the fastest of the forms measured here: grid-stride and blocked,
+// 4-16 loads, 2-32 workgroups per CU); a word is written only if the XOR of
+// everything read hits a constant, so the loads cannot be dropped.
+__global__ __launch_bounds__(256) void probe_stream_kernel(const uint4* __restrict__ p, long long n4,
+                                                           unsigned* __restrict__ out) {
+    const long long per = (n4 + gridDim.x - 1) / gridDim.x;
+    const long long b0 = (long long)blockIdx.x * per, b1 = b0 + per < n4 ? b0 + per : n4;
+    long long i = b0 + threadIdx.x;
+    unsigned x = 0;
+    for (; i + 7 * 256 < b1; i += 8 * 256) {
+        uint4 v[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) v[u] = p[i + u * 256];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) x ^= v[u].x ^ v[u].y ^ v[u].z ^ v[u].w;
+    }
+    for (; i < b1; i += 256) {
+        const uint4 v = p[i];
+        x ^= v.x ^ v.y ^ v.z ^ v.w;
+    }
+    if (x == 0x9e3779b9u) out[blockIdx.x] = x;
+}
+
+// FP64 FMA throughput: eight independent chains per lane; the result is
+// written only if it hits a constant.
+__global__ __launch_bounds__(256) void probe_fma64_kernel(int iters, double seed, double* __restrict__ out) {
+    double a[8];
+#pragma unroll
+    for (int k = 0; k < 8; ++k) a[k] = seed + 1e-9 * threadIdx.x + k;
+    const double m = 1.0000000001, c = 1e-12;
+#pragma unroll 4
+    for (int it = 0; it < iters; ++it) {
+#pragma unroll
+        for (int k = 0; k < 8; ++k) a[k] = fma(a[k], m, c);
+    }
+    double s = 0;
+#pragma unroll
+    for (int k = 0; k < 8; ++k) s += a[k];
+    if (s == 12345.678) out[blockIdx.x] = s;
+}
+
 extern "C" {
 
 const char* pq3d_last_error(void) { return g_err.c_str(); }
@@ -177,6 +225,83 @@ int pq3d_event_elapsed_ms(void* start, void* stop, float* ms) {
     clear_error();
     PQ_HIP(hipEventSynchronize((hipEvent_t)stop));
     PQ_HIP(hipEventElapsedTime(ms, (hipEvent_t)start, (hipEvent_t)stop));
+    return 0;
+}
+
+// Sustained read bandwidth over a device buffer of `bytes` (allocated here,
+// rounded down to 16 bytes): the fastest of `reps` timed passes after one
+// untimed pass; GB/s (1e9 bytes) and its ms.  Workgroups: 32 per CU.
+int pq3d_probe_stream(long long bytes, int reps, float* ms_per, double* gbs) {
+    clear_error();
+    PQ_CHECK(bytes >= 16 && reps > 0 && ms_per && gbs, "pq3d_probe_stream: bad arguments");
+    int dev = 0;
+    PQ_HIP(hipGetDevice(&dev));
+    hipDeviceProp_t p;
+    PQ_HIP(hipGetDeviceProperties(&p, dev));
+    const long long n4 = bytes / 16;
+    const unsigned blocks = (unsigned)p.multiProcessorCount * 32;
+    void* buf = nullptr;
+    unsigned* out = nullptr;
+    PQ_HIP(hipMalloc(&buf, (size_t)n4 * 16));
+    hipError_t err = hipMalloc((void**)&out, blocks * sizeof(unsigned));
+    hipEvent_t a = nullptr, b = nullptr;
+    if (err == hipSuccess) err = hipMemset(buf, 0, (size_t)n4 * 16);
+    if (err == hipSuccess) err = hipEventCreate(&a);
+    if (err == hipSuccess) err = hipEventCreate(&b);
+    float best = 0;
+    for (int r = -1; r < reps && err == hipSuccess; ++r) {
+        err = hipEventRecord(a, 0);
+        hipLaunchKernelGGL(probe_stream_kernel, dim3(blocks), dim3(256), 0, 0, (const uint4*)buf, n4, out);
+        if (err == hipSuccess) err = hipGetLastError();
+        if (err == hipSuccess) err = hipEventRecord(b, 0);
+        if (err == hipSuccess) err = hipEventSynchronize(b);
+        float ms = 0;
+        if (err == hipSuccess) err = hipEventElapsedTime(&ms, a, b);
+        if (r >= 0 && (best == 0 || ms < best)) best = ms;
+    }
+    if (a) (void)hipEventDestroy(a);
+    if (b) (void)hipEventDestroy(b);
+    (void)hipFree(buf);
+    if (out) (void)hipFree(out);
+    PQ_HIP(err);
+    *ms_per = best;
+    *gbs = (double)n4 * 16 / (best * 1e-3) / 1e9;
+    return 0;
+}
+
+// Sustained FP64 FMA rate: `iters` FMAs in each of eight chains per lane,
+// 256-thread workgroups, 64 per CU; TFLOP/s (an FMA = 2 flops) and ms per
+// launch over `reps` timed launches after one untimed.
+int pq3d_probe_fma64(int iters, int reps, float* ms_per, double* tflops) {
+    clear_error();
+    PQ_CHECK(iters > 0 && reps > 0 && ms_per && tflops, "pq3d_probe_fma64: bad arguments");
+    int dev = 0;
+    PQ_HIP(hipGetDevice(&dev));
+    hipDeviceProp_t p;
+    PQ_HIP(hipGetDeviceProperties(&p, dev));
+    const unsigned blocks = (unsigned)p.multiProcessorCount * 64;
+    double* out = nullptr;
+    PQ_HIP(hipMalloc((void**)&out, blocks * sizeof(double)));
+    hipEvent_t a = nullptr, b = nullptr;
+    hipError_t err = hipEventCreate(&a);
+    if (err == hipSuccess) err = hipEventCreate(&b);
+    float ms = 0;
+    if (err == hipSuccess) {
+        for (int r = -1; r < reps; ++r) {
+            if (r == 0) (void)hipEventRecord(a, 0);
+            hipLaunchKernelGGL(probe_fma64_kernel, dim3(blocks), dim3(256), 0, 0, iters, 1.0 + r, out);
+        }
+        err = hipGetLastError();
+        if (err == hipSuccess) err = hipEventRecord(b, 0);
+        if (err == hipSuccess) err = hipEventSynchronize(b);
+        if (err == hipSuccess) err = hipEventElapsedTime(&ms, a, b);
+    }
+    if (a) (void)hipEventDestroy(a);
+    if (b) (void)hipEventDestroy(b);
+    (void)hipFree(out);
+    PQ_HIP(err);
+    *ms_per = ms / reps;
+    *tflops = 2.0 * 8.0 * iters * (double)blocks * 256 / (*ms_per * 1e-3) / 1e12;
     return 0;
 }
 
