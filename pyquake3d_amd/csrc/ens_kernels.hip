@@ -597,15 +597,18 @@ __device__ __forceinline__ void ens_dp45(const EElem& e, double V0, double eta, 
         t[i] = y[i] + h * ((9017. / 3168.) * k1[i] + (-355. / 33.) * k2[i] + (-46732. / 5247.) * k3[i] +
                            (49. / 176.) * k4[i] + (-5103. / 18656.) * k5[i]);
     ens_rhs(e, V0, eta, t, k6);
+    // y4 takes its k1..k6 terms before the seventh stage, so that k1..k6 are
+    // not held through it
 #pragma unroll
-    for (int i = 0; i < 4; ++i)
+    for (int i = 0; i < 4; ++i) {
         y5[i] = y[i] + h * ((35. / 384.) * k1[i] + (500. / 1113.) * k3[i] + (125. / 192.) * k4[i] +
                             (-2187. / 6784.) * k5[i] + (11. / 84.) * k6[i]);
+        y4[i] = (5179. / 57600.) * k1[i] + (7571. / 16695.) * k3[i] + (393. / 640.) * k4[i] +
+                (-92097. / 339200.) * k5[i] + (187. / 2100.) * k6[i];
+    }
     ens_rhs(e, V0, eta, y5, k7);
 #pragma unroll
-    for (int i = 0; i < 4; ++i)
-        y4[i] = y[i] + h * ((5179. / 57600.) * k1[i] + (7571. / 16695.) * k3[i] + (393. / 640.) * k4[i] +
-                            (-92097. / 339200.) * k5[i] + (187. / 2100.) * k6[i] + (1. / 40.) * k7[i]);
+    for (int i = 0; i < 4; ++i) y4[i] = y[i] + h * (y4[i] + (1. / 40.) * k7[i]);
 }
 
 __device__ __forceinline__ double epy_max(double a, double b) { return b > a ? b : a; }
@@ -683,7 +686,7 @@ __global__ __launch_bounds__(256) void ens_rates_kernel(EnsArgs A) {
 
 // One attempt for every item of a pending member; round 0 first sums the
 // GEMM's split-K partials into the frozen traction rates.
-__global__ __launch_bounds__(256) void ens_attempt_kernel(EnsArgs A, int round) {
+__global__ __launch_bounds__(256, 5) void ens_attempt_kernel(EnsArgs A, int round) {
     const long long n = A.n;
     const int B = A.B;
     const long long items = n * B;
