@@ -352,18 +352,32 @@ __device__ __forceinline__ void ens_gemm3_half(const float4 (&a)[NM][8], const b
     }
 }
 
-// NM matrices per half: the strike half multiplies S[0..NM) by u1, the dip
-// half D[0..NM) by u2, both into output components Q0 .. Q0 + NM - 1.  The
-// shear pass is NM = 2 (A1s, A2s | A1d, A2d -> q 0, 1), the normal pass NM = 1
-// (Bs | Bd -> q 2).  ZERO_NORMAL: the shear pass also writes the normal
-// partials as zeros (Bs = Bd = 0 or fix_Tn: no normal pass runs).
-template <int L, int NM, int Q0, bool ZERO_NORMAL>
+// NM matrix slots per half: the strike half multiplies S[0..NM) by u1, the
+// dip half D[0..NM) by u2.  The shear pass (ROWPAIR false) is A1s, A2s |
+// A1d, A2d of row tile blockIdx.x into components 0, 1.  The normal pass
+// (ROWPAIR true) is Bs | Bd of the two row tiles 2 blockIdx.x and
+// 2 blockIdx.x + 1 into component 2: the same register footprint as the
+// shear pass, and each member-operand load serves two matrix blocks instead
+// of one.  ZERO_NORMAL: the shear pass also writes the normal partials as
+// zeros (Bs = Bd = 0 or fix_Tn: no normal pass runs).
+template <int L, int NM, bool ZERO_NORMAL, bool ROWPAIR>
 __device__ __forceinline__ void ens_gemm3_tile(const float* const (&S)[NM], const float* const (&D)[NM],
                                                long long KB, long long n, int kc, const uint4* __restrict__ planes,
                                                long long plane_len, float* __restrict__ partial,
                                                long long npad) {
+    static_assert(!ROWPAIR || NM == 2, "the normal pass pairs two row tiles");
     const int lane = threadIdx.x & 63;
-    const long long r0 = (long long)blockIdx.x * 64;
+    const long long ntiles = (n + 63) / 64;
+    // row tile of slot q; a pair's second tile past the last one re-reads the
+    // first (in bounds) and writes nothing
+    long long rt[NM];
+    bool wr[NM];
+#pragma unroll
+    for (int q = 0; q < NM; ++q) {
+        const long long t = ROWPAIR ? 2 * (long long)blockIdx.x + q : (long long)blockIdx.x;
+        wr[q] = t < ntiles;
+        rt[q] = wr[q] ? t : 2 * (long long)blockIdx.x;
+    }
     const long long k0 = (long long)blockIdx.y * kc;
     long long k1 = k0 + kc;
     if (k1 > n) k1 = n;
@@ -377,20 +391,25 @@ __device__ __forceinline__ void ens_gemm3_tile(const float* const (&S)[NM], cons
             for (int j = 0; j < 4; ++j) acc[q][i][j] = v4f{0.f, 0.f, 0.f, 0.f};
     float4 as[NM][8], ad[NM][8];
     bf8 us[4][3], ud[4][3];
-    // A from the panel copies: the 64-row x 32-source block (this row tile,
-    // source block kb / 32) is 8 KB contiguous, float4 t * 64 + lane holding
-    // rows 4 lr .. 4 lr + 3 of source kb + 8 g + t, so every load instruction
-    // reads 1 KB contiguously.  Rows and sources past n are stored as zeros.
+    // A from the panel copies: the 64-row x 32-source block of one matrix
+    // (this row tile, source block kb / 32) is 8 KB contiguous, float4
+    // t * 64 + lane holding rows 4 lr .. 4 lr + 3 of source kb + 8 g + t, so
+    // every load instruction reads 1 KB contiguously.  The 2 NM matrices of a
+    // pass are interleaved block by block (S[0..NM) then D[0..NM), 8 KB
+    // apart), so a wave reads one contiguous stream of 16 NM KB per step.
+    // Rows and sources past n are stored as zeros.
+    // (the normal pass's buffer interleaves Bs and Bd only)
+    constexpr long long BSTRIDE = (ROWPAIR ? 2 : 2 * NM) * 2048;  // floats per (row tile, source block)
     const float* Pm[2][NM];
 #pragma unroll
     for (int q = 0; q < NM; ++q) {
-        Pm[0][q] = S[q] + (size_t)blockIdx.x * KB * 2048 + 4 * lane;
-        Pm[1][q] = D[q] + (size_t)blockIdx.x * KB * 2048 + 4 * lane;
+        Pm[0][q] = S[q] + (size_t)rt[q] * KB * BSTRIDE + 4 * lane;
+        Pm[1][q] = D[q] + (size_t)rt[q] * KB * BSTRIDE + 4 * lane;
     }
     auto load_a = [&](float4 (&a)[NM][8], int half, long long kb) {
 #pragma unroll
         for (int q = 0; q < NM; ++q) {
-            const float* B = Pm[half][q] + (size_t)(kb >> 5) * 2048;
+            const float* B = Pm[half][q] + (size_t)(kb >> 5) * BSTRIDE;
 #pragma unroll
             for (int t = 0; t < 8; ++t) a[q][t] = *reinterpret_cast<const float4*>(B + 256 * t);
         }
@@ -428,19 +447,23 @@ __device__ __forceinline__ void ens_gemm3_tile(const float* const (&S)[NM], cons
     }
     // C fragment as in ens_gemm_tile
 #pragma unroll
-    for (int q = 0; q < NM; ++q)
+    for (int q = 0; q < NM; ++q) {
+        if (!wr[q]) continue;
+        const int comp = ROWPAIR ? 2 : q;
 #pragma unroll
         for (int j = 0; j < 4; ++j) {
             const int member = 4 * lr + j;
 #pragma unroll
             for (int v = 0; v < 4; ++v) {
-                const long long prow = r0 + 4 * (4 * g + v);
+                const long long prow = rt[q] * 64 + 4 * (4 * g + v);
                 const float4 o = make_float4(acc[q][0][j][v], acc[q][1][j][v], acc[q][2][j][v], acc[q][3][j][v]);
-                *reinterpret_cast<float4*>(partial + (((size_t)blockIdx.y * 3 + Q0 + q) * 64 + member) * npad +
+                *reinterpret_cast<float4*>(partial + (((size_t)blockIdx.y * 3 + comp) * 64 + member) * npad +
                                            prow) = o;
             }
         }
+    }
     if (ZERO_NORMAL) {
+        const long long r0 = (long long)blockIdx.x * 64;
 #pragma unroll
         for (int j = 0; j < 4; ++j) {
             const int member = 4 * lr + j;
@@ -464,15 +487,17 @@ __device__ __forceinline__ void ens_gemm3_pass(const float* __restrict__ A1s, co
     if (!with_normal) {
         const float* const S[2] = {A1s, A2s};
         const float* const D[2] = {A1d, A2d};
-        ens_gemm3_tile<L, 2, 0, true>(S, D, KB, n, kc, planes, plane_len, partial, npad);
+        ens_gemm3_tile<L, 2, true, false>(S, D, KB, n, kc, planes, plane_len, partial, npad);
     } else if (blockIdx.z == 0) {
         const float* const S[2] = {A1s, A2s};
         const float* const D[2] = {A1d, A2d};
-        ens_gemm3_tile<L, 2, 0, false>(S, D, KB, n, kc, planes, plane_len, partial, npad);
+        ens_gemm3_tile<L, 2, false, false>(S, D, KB, n, kc, planes, plane_len, partial, npad);
     } else {
-        const float* const S[1] = {Bs};
-        const float* const D[1] = {Bd};
-        ens_gemm3_tile<L, 1, 2, false>(S, D, KB, n, kc, planes, plane_len, partial, npad);
+        // normal pass over row-tile pairs: half of this grid plane has no pair
+        if (2 * (long long)blockIdx.x >= (n + 63) / 64) return;
+        const float* const S[2] = {Bs, Bs};
+        const float* const D[2] = {Bd, Bd};
+        ens_gemm3_tile<L, 2, false, true>(S, D, KB, n, kc, planes, plane_len, partial, npad);
     }
 }
 
@@ -506,11 +531,13 @@ __global__ __launch_bounds__(64) void ens_gemm3_kernel(const float* __restrict__
 
 // Panel copy of one source-major FP32 matrix (leading dimension ld): for row
 // tile R (64 rows) and source block K (32 sources), 2048 floats at
-// (R * KB + K) * 2048, float4 t * 64 + lane = rows 64 R + 4 (lane % 16) + 0..3
-// of source 32 K + 8 (lane / 16) + t; rows and sources past n are zero.  One
-// thread per float4 of the panel.
+// (R * KB + K) * bstride4 * 4 from P (P is the matrix's slot in a buffer
+// that interleaves the blocks of several matrices), float4 t * 64 + lane =
+// rows 64 R + 4 (lane % 16) + 0..3 of source 32 K + 8 (lane / 16) + t; rows
+// and sources past n are zero.  One thread per float4 of the panel.
 __global__ __launch_bounds__(256) void ens_panel_kernel(const float* __restrict__ M, long long ld, long long n,
-                                                        long long KB, long long total4, float4* __restrict__ P) {
+                                                        long long KB, long long total4, long long bstride4,
+                                                        float4* __restrict__ P) {
     const long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (idx >= total4) return;
     const int w = (int)(idx & 511);
@@ -525,7 +552,7 @@ __global__ __launch_bounds__(256) void ens_panel_kernel(const float* __restrict_
         for (int i = 0; i < 4; ++i)
             if (row + i < n) v[i] = M[(size_t)src * ld + row + i];
     }
-    P[idx] = make_float4(v[0], v[1], v[2], v[3]);
+    P[blk * bstride4 + w] = make_float4(v[0], v[1], v[2], v[3]);
 }
 
 // ---------------------------------------------------------------------------
@@ -852,7 +879,11 @@ struct Ensemble {
     uint4* planes = nullptr;  // u's BF16 parts: 2 arrays x 3 parts x (npad / 8) x 64 x 16 bytes
     // panel copies of the matrices the split GEMM reads (ens_panel_kernel
     // layout, npad x KB * 32 floats each), built when the structure is scanned
-    // or set and when the precision or fix_Tn changes
+    // or set and when the precision or fix_Tn changes.  The shear pass's four
+    // (A1s, A2s, A1d, A2d) share one buffer with their blocks interleaved,
+    // and the normal pass's two (Bs, Bd) another; panel[k] is matrix k's slot.
+    float* pan_shear = nullptr;
+    float* pan_normal = nullptr;
     float* panel[6] = {nullptr};
     long long KB = 0;         // 32-source blocks per row tile
     int panels_ready = 0;     // the panels the split GEMM needs hold the current matrices
@@ -910,16 +941,27 @@ static int ens_update_panels(Ensemble* e) {
     e->panels_ready = 0;
     const size_t bytes = ens_panel_bytes(e);
     const long long total4 = (long long)(bytes / sizeof(float4));
+    const bool need_shear = e->prec == 1, need_normal = e->prec == 1 && !ens_no_normal(e);
+    if (!need_shear && e->pan_shear) {
+        (void)hipFree(e->pan_shear);
+        e->pan_shear = nullptr;
+    }
+    if (!need_normal && e->pan_normal) {
+        (void)hipFree(e->pan_normal);
+        e->pan_normal = nullptr;
+    }
+    if (need_shear && !e->pan_shear) PQ_HIP(hipMalloc((void**)&e->pan_shear, 4 * bytes));
+    if (need_normal && !e->pan_normal) PQ_HIP(hipMalloc((void**)&e->pan_normal, 2 * bytes));
+    // slot of matrix k: shear buffer A1s 0, A2s 1, A1d 2, A2d 3; normal buffer Bs 0, Bd 1
+    const int slot[6] = {0, 1, 0, 2, 3, 1};
     for (int k = 0; k < 6; ++k) {
-        const bool need = e->prec == 1 && ((k != 2 && k != 5) || !ens_no_normal(e));
-        if (!need) {
-            if (e->panel[k]) (void)hipFree(e->panel[k]);
-            e->panel[k] = nullptr;
-            continue;
-        }
-        if (!e->panel[k]) PQ_HIP(hipMalloc((void**)&e->panel[k], bytes));
+        const bool normal = k == 2 || k == 5;
+        float* buf = normal ? e->pan_normal : e->pan_shear;
+        e->panel[k] = buf ? buf + (size_t)slot[k] * 2048 : nullptr;
+        if (!buf) continue;
+        const long long bstride4 = (normal ? 2 : 4) * 512;
         hipLaunchKernelGGL(ens_panel_kernel, dim3((unsigned)((total4 + 255) / 256)), dim3(256), 0, e->stream,
-                           e->mats[k], e->ld, e->n, e->KB, total4, (float4*)e->panel[k]);
+                           e->mats[k], e->ld, e->n, e->KB, total4, bstride4, (float4*)e->panel[k]);
         PQ_HIP(hipGetLastError());
     }
     PQ_HIP(hipStreamSynchronize(e->stream));
@@ -1054,7 +1096,8 @@ int pq3d_ens_destroy(void* h) {
     (void)hipFree(e->f);
     (void)hipFree(e->U);
     (void)hipFree(e->planes);
-    for (int k = 0; k < 6; ++k) (void)hipFree(e->panel[k]);
+    (void)hipFree(e->pan_shear);
+    (void)hipFree(e->pan_normal);
     (void)hipFree(e->partial);
     (void)hipFree(e->mc);
     (void)hipFree(e->slot);
