@@ -129,6 +129,58 @@ def time_ensemble(ens, steps, warmup, graph_steps):
     return ms, int(np.sum(mem["steps"] - before))
 
 
+def rupture_ensemble_leg(sim, xg, timed=200, graph_steps=8, vdyn=1e-2, min_members=8, chunk=200,
+                         max_steps=12000):
+    """The batched rate while members rupture.  The metric's members creep
+    (their nucleation size exceeds the patch), so this leg runs 64 members
+    with a - b in [-0.026, -0.022] (nucleation size near 7 km) on the same
+    mesh and matrices, steps them until at least `min_members` members have
+    slip rates above `vdyn` (coseismic: steps shrink, members desynchronise,
+    rejections make the second attempt round live), then times `timed`
+    global steps there."""
+    from pyquake3d_amd import synth
+    from pyquake3d_amd.ensemble import Ensemble, init_members
+    mcols = np.stack([synth.member_params(xg, m, base_seed=5000, amb_range=(-0.026, -0.022))
+                      for m in range(MEMBERS)])
+    ens = Ensemble(len(xg), MEMBERS, matrices_from=sim)
+    init_members(ens, mcols, V0, VPL, MU, CS)
+    try:
+        return time_ensemble_rupture(ens, vdyn, min_members, chunk, max_steps, timed, graph_steps)
+    finally:
+        ens.close()
+
+
+def time_ensemble_rupture(ens, vdyn=1e-2, min_members=8, chunk=200, max_steps=12000, timed=200, graph_steps=8):
+    """Step the ensemble on until at least `min_members` members have slip
+    rates above `vdyn`, then time `timed` global steps there."""
+    searched = 0
+    while searched < max_steps:
+        vmax = ens.get("slipv").max(axis=1)
+        if int(np.sum(vmax > vdyn)) >= min_members:
+            break
+        ens.run(chunk, graph_steps)
+        searched += chunk
+    vmax0 = ens.get("slipv").max(axis=1)
+    before = ens.members()
+    t0 = time.perf_counter()
+    ens.run(timed, graph_steps)
+    ms = (time.perf_counter() - t0) * 1e3
+    after = ens.members()
+    vmax1 = ens.get("slipv").max(axis=1)
+    acc = int(np.sum(after["steps"] - before["steps"]))
+    return {"value": acc / (ms * 1e-3), "unit": "member-steps/s", "global_steps": timed,
+            "ms_per_global_step": ms / timed, "steps_before_window": searched,
+            "members_above_vdyn_at_start": int(np.sum(vmax0 > vdyn)),
+            "members_above_vdyn_at_end": int(np.sum(vmax1 > vdyn)), "vdyn_m_per_s": vdyn,
+            "peak_slip_rate_m_per_s": float(max(vmax0.max(), vmax1.max())),
+            "min_step_s": float(min(before["h"].min(), after["h"].min())),
+            "failed_members": int(np.sum(after["status"] != 0)),
+            "members": "64 members, a - b uniform in [-0.026, -0.022] on the patch (seed 5000 + member), Dc "
+                       "lognormal (median 0.14 m, sigma 0.3), normal stress uniform 20-30 MPa; same mesh and "
+                       "FP32 matrices, BF16-split GEMM",
+            "note": "a member-step is one accepted step of one member; the window starts once members rupture"}
+
+
 def serve(n_requests, rate, seed=7):
     """Continuous-batching serving over 2k, 10k and 30k planar meshes."""
     from pyquake3d_amd import readmsh, synth
@@ -564,6 +616,8 @@ def main():
                     help="Poisson arrival rate, requests/s over the three meshes (40/s keeps the 30k mesh's "
                          "64 slots more than half full)")
     ap.add_argument("--serve-only", action="store_true", help="run only the serving leg (JSON of that leg)")
+    ap.add_argument("--rupture-steps", type=int, default=200,
+                    help="global steps timed once members rupture (0 skips that window)")
     ap.add_argument("--no-hmat", action="store_true", help="skip the 60k H-matrix leg")
     ap.add_argument("--no-config1", action="store_true", help="skip the 1,600-element 2,000-step leg")
     ap.add_argument("--hmat-only", action="store_true", help="run only the 60k H-matrix leg (JSON of that leg)")
@@ -660,6 +714,12 @@ def main():
                 "gemm_ms": gemm_ms, "gemm_precision": precision, "gemm_bf16_split": split,
                 "member_sim_time_s_min": float(mem["time"].min()),
                 "member_sim_time_s_max": float(mem["time"].max()), "failed_members": int((mem["status"] != 0).sum())}
+    if args.rupture_steps > 0:
+        ens_info["rupture"] = rupture_ensemble_leg(sim, xg, timed=args.rupture_steps, graph_steps=args.graph_steps)
+        r = ens_info["rupture"]
+        log(f"[bench] ensemble in rupture: {r['value']:.1f} member-steps/s over {r['global_steps']} global steps "
+            f"after {r['steps_before_window']} more; members above Vdyn {r['members_above_vdyn_at_start']} -> "
+            f"{r['members_above_vdyn_at_end']}, peak {r['peak_slip_rate_m_per_s']:.3g} m/s")
     # traction rates of the members' current slip rates through the batched
     # path (FP32 storage, MFMA GEMM), checked below against the FP64 product
     ens.gemm()

@@ -143,13 +143,19 @@ def friction_params(xg_m, dc_sigma=0.0, seed=0, a_vw=0.004, a_vs=0.04, b=0.03, d
     return np.stack([np.full(n, rake), a, bb, dcv, np.full(n, F0), tt, sn, V, np.zeros(n), np.zeros(n)], axis=1)
 
 
-def member_params(xg_m, member, base_seed=1000):
-    """One ensemble member's friction parameters (seed = base + member)."""
+def member_params(xg_m, member, base_seed=1000, amb_range=(-0.006, -0.002)):
+    """One ensemble member's friction parameters (seed = base + member):
+    a - b uniform in ``amb_range`` on the velocity-weakening patch, Dc
+    lognormal (median 0.14 m, sigma 0.3), normal stress uniform 20-30 MPa.
+    With the default range the critical nucleation size (about
+    mu Dc / ((b - a) sigma), 30-90 km) exceeds the 30 x 15 km patch, so the
+    members creep without seismic events; a range near -0.024 puts it near
+    7 km and the members rupture."""
     rng = np.random.default_rng(base_seed + member)
     n = len(xg_m)
     d = patch_distance_km(xg_m)
     b = np.full(n, 0.03)
-    amb = rng.uniform(-0.006, -0.002)
+    amb = rng.uniform(*amb_range)
     a_vw = 0.03 + amb
     a = a_vw + (0.04 - a_vw) * np.clip(d / TAPER_KM, 0.0, 1.0)
     dc = 0.14 * np.exp(0.3 * rng.standard_normal(n))
