@@ -133,14 +133,14 @@ def rupture_ensemble_leg(sim, xg, timed=200, graph_steps=8, vdyn=1e-2, min_membe
                          max_steps=12000):
     """The batched rate while members rupture.  The metric's members creep
     (their nucleation size exceeds the patch), so this leg runs 64 members
-    with a - b in [-0.026, -0.022] (nucleation size near 7 km) on the same
+    with a - b in [-0.026, -0.022] (nucleation size near 7 km), rake 0.5, on the same
     mesh and matrices, steps them until at least `min_members` members have
     slip rates above `vdyn` (coseismic: steps shrink, members desynchronise,
     rejections make the second attempt round live), then times `timed`
     global steps there."""
     from pyquake3d_amd import synth
     from pyquake3d_amd.ensemble import Ensemble, init_members
-    mcols = np.stack([synth.member_params(xg, m, base_seed=5000, amb_range=(-0.026, -0.022))
+    mcols = np.stack([synth.member_params(xg, m, base_seed=5000, amb_range=(-0.026, -0.022), rake=0.5)
                       for m in range(MEMBERS)])
     ens = Ensemble(len(xg), MEMBERS, matrices_from=sim)
     init_members(ens, mcols, V0, VPL, MU, CS)
@@ -176,8 +176,8 @@ def time_ensemble_rupture(ens, vdyn=1e-2, min_members=8, chunk=200, max_steps=12
             "min_step_s": float(min(before["h"].min(), after["h"].min())),
             "failed_members": int(np.sum(after["status"] != 0)),
             "members": "64 members, a - b uniform in [-0.026, -0.022] on the patch (seed 5000 + member), Dc "
-                       "lognormal (median 0.14 m, sigma 0.3), normal stress uniform 20-30 MPa; same mesh and "
-                       "FP32 matrices, BF16-split GEMM",
+                       "lognormal (median 0.14 m, sigma 0.3), normal stress uniform 20-30 MPa, rake 0.5; same "
+                       "mesh and FP32 matrices, BF16-split GEMM",
             "note": "a member-step is one accepted step of one member; the window starts once members rupture"}
 
 

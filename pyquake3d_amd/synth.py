@@ -143,10 +143,11 @@ def friction_params(xg_m, dc_sigma=0.0, seed=0, a_vw=0.004, a_vs=0.04, b=0.03, d
     return np.stack([np.full(n, rake), a, bb, dcv, np.full(n, F0), tt, sn, V, np.zeros(n), np.zeros(n)], axis=1)
 
 
-def member_params(xg_m, member, base_seed=1000, amb_range=(-0.006, -0.002)):
+def member_params(xg_m, member, base_seed=1000, amb_range=(-0.006, -0.002), dc_sigma=0.3, rake=0.0):
     """One ensemble member's friction parameters (seed = base + member):
     a - b uniform in ``amb_range`` on the velocity-weakening patch, Dc
-    lognormal (median 0.14 m, sigma 0.3), normal stress uniform 20-30 MPa.
+    lognormal per element (median 0.14 m, ``dc_sigma``), normal stress
+    uniform 20-30 MPa, slip direction ``rake`` (radians from strike).
     With the default range the critical nucleation size (about
     mu Dc / ((b - a) sigma), 30-90 km) exceeds the 30 x 15 km patch, so the
     members creep without seismic events; a range near -0.024 puts it near
@@ -158,11 +159,11 @@ def member_params(xg_m, member, base_seed=1000, amb_range=(-0.006, -0.002)):
     amb = rng.uniform(*amb_range)
     a_vw = 0.03 + amb
     a = a_vw + (0.04 - a_vw) * np.clip(d / TAPER_KM, 0.0, 1.0)
-    dc = 0.14 * np.exp(0.3 * rng.standard_normal(n))
+    dc = 0.14 * np.exp(dc_sigma * rng.standard_normal(n))
     sn = np.full(n, rng.uniform(20.0, 30.0))
     V = np.full(n, VPL)
     tt = sn * a * np.arcsinh(V / (2 * V0) * np.exp((F0 + b * np.log(V0 / VPL)) / a))
-    return np.stack([np.zeros(n), a, b, dc, np.full(n, F0), tt, sn, V, np.zeros(n), np.zeros(n)], axis=1)
+    return np.stack([np.full(n, rake), a, b, dc, np.full(n, F0), tt, sn, V, np.zeros(n), np.zeros(n)], axis=1)
 
 
 def write_param_file(fname, cols):
