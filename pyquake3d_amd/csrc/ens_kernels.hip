@@ -37,6 +37,7 @@
 //      AdotV per step, QDsim_gpu.py:946-957), so the member's trajectory is
 //      exactly that of the reference's retry loop.
 #include "common.hpp"
+#include "rsf_math.hpp"
 
 #include <algorithm>
 #include <cmath>
@@ -575,16 +576,18 @@ __device__ __forceinline__ void elem_constants(EElem& e, double V0) {
 
 // The aging-law RHS with radiation damping (QDsim_gpu.py:830-891): the same
 // formulas as the dense path's, with the per-element quotients taken once per
-// attempt, one reciprocal of sigma per stage, and exp(-c) as 1/exp(c).
+// attempt, one reciprocal of sigma per stage, exp(-c) as 1/exp(c), and the
+// exponentials (arguments clipped to +-700, as the reference clips them) and
+// reciprocals from rsf_math.hpp.
 __device__ __forceinline__ void ens_rhs(const EElem& e, double V0, double eta, const double* y, double* k) {
     const double tn = y[0] * 1e6, t1 = y[1] * 1e6, t2 = y[2] * 1e6, st = y[3];
     const double sg = tn - e.P;
-    const double isg = 1.0 / sg;
+    const double isg = rcp_fast(sg);
     const double ias = e.ia * isg;  // 1 / (a sigma)
-    const double ex = exp(eclip(-st * e.ia));
+    const double ex = exp_clipped(eclip(-st * e.ia));
     const double c1 = eclip(t1 * ias), c2 = eclip(t2 * ias);
-    const double ep1 = exp(c1), ep2 = exp(c2);
-    const double em1 = 1.0 / ep1, em2 = 1.0 / ep2;
+    const double ep1 = exp_clipped(c1), ep2 = exp_clipped(c2);
+    const double em1 = rcp_fast(ep1), em2 = rcp_fast(ep2);
     const double ch1 = (ep1 + em1) * 0.5, ch2 = (ep2 + em2) * 0.5;
     const double sh1 = (ep1 - em1) * 0.5, sh2 = (ep2 - em2) * 0.5;
     const double qx = 2 * V0 * ias * ex;  // 2 V0 exp(-psi/a) / (a sigma)
@@ -593,11 +596,11 @@ __device__ __forceinline__ void ens_rhs(const EElem& e, double V0, double eta, c
     const double dV2dsig = -qx * (t2 * isg) * ch2;
     const double dV1dst = -e.c2v0a * ex * sh1;
     const double dV2dst = -e.c2v0a * ex * sh2;
-    const double dstdt = e.bdc * (V0 * exp(eclip((e.f0 - st) * e.ib)) - e.V);
+    const double dstdt = e.bdc * (V0 * exp_clipped(eclip((e.f0 - st) * e.ib)) - e.V);
     const double ds = e.dsig - e.dPdt;
     k[0] = e.dsig * 1e-6;
-    k[1] = (-e.A1 + e.sl - eta * (dV1dsig * ds + dV1dst * dstdt)) / (1.0 + eta * dV1dtau) * 1e-6;
-    k[2] = (-e.A2 + e.sl - eta * (dV2dsig * ds + dV2dst * dstdt)) / (1.0 + eta * dV2dtau) * 1e-6;
+    k[1] = (-e.A1 + e.sl - eta * (dV1dsig * ds + dV1dst * dstdt)) * rcp_fast(1.0 + eta * dV1dtau) * 1e-6;
+    k[2] = (-e.A2 + e.sl - eta * (dV2dsig * ds + dV2dst * dstdt)) * rcp_fast(1.0 + eta * dV2dtau) * 1e-6;
     k[3] = dstdt;
 }
 
