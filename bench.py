@@ -16,6 +16,14 @@ Legs
   six leading products in FP32 on the BF16 MFMA units (the same ensemble with
   the FP32-MFMA GEMM is reported beside it).
   value = accepted member-steps per second (64 x global steps / time).
+* ensemble.rupture: the same mesh and matrices with 64 members that rupture
+  (a-b ~ U[-0.026,-0.022], rake 0.5: nucleation size near 7 km, while the
+  value's members creep), timed over 200 global steps once eight members
+  are coseismic (slip rate above 1e-2 m/s).
+* serving: Poisson arrivals over 2k/10k/30k planar meshes; p50/p99
+  time-to-result, member-steps/s and slot occupancy.
+* config1: the 1,600-element case, 2,000 adaptive steps on the plain FP64
+  path, beside the NumPy restatement on the host.
 * plain: one simulation (aging law, Dc lognormal sigma 0.2, V = Vpl = 1e-9
   m/s, steady state) with FP32 and with FP64 matrix storage; member-steps/s.
 * roofline: the ensemble GEMM (dominant kernel of `value`): with the BF16
