@@ -72,7 +72,8 @@ enum EField : int {
     E_ADOTV1, E_ADOTV2, E_DSIGMADT, E_COSR0, E_SINR0, E_Y5_0, E_Y5_1, E_Y5_2, E_Y5_3, E_NFIELDS
 };
 
-constexpr int ENS_ROUNDS = 2;  // attempt rounds per global step (even: member control parity repeats)
+constexpr int ENS_ROUNDS = 4;  // attempt rounds per global step (even: member control parity repeats)
+constexpr int ENS_LATER_BLOCKS = 1024;  // grid of the rounds after the first (pending members only)
 
 struct MemberCtrl {
     double h;            // step size of the next attempt
@@ -714,59 +715,133 @@ __global__ __launch_bounds__(256) void ens_rates_kernel(EnsArgs A) {
         A.fix_tn ? f[(size_t)E_NORMAL_LOADING * items] : s2 + f[(size_t)E_NORMAL_LOADING * items];
 }
 
+// The members (of B <= 64) whose flag is set, in order, into list[0..count):
+// wave 0 of the workgroup ballots the flags; the caller synchronises.
+__device__ __forceinline__ void member_list(bool flag, int* list, int* count) {
+    if (threadIdx.x < 64) {
+        const unsigned long long bal = __ballot(flag);
+        if (flag) list[__popcll(bal & ((1ull << threadIdx.x) - 1ull))] = (int)threadIdx.x;
+        if (threadIdx.x == 0) *count = __popcll(bal);
+    }
+}
+
+// One Dormand-Prince attempt of item tt (member m): the candidate state into
+// the Y5 fields, the two error measures' bit patterns into r1b, r2b.
+__device__ __forceinline__ void attempt_item(const EnsArgs& A, int round, const MemberCtrl* mc, long long tt,
+                                             int m, unsigned long long& r1b, unsigned long long& r2b) {
+    const long long n = A.n;
+    const int B = A.B;
+    const long long items = n * B;
+    double* f = A.f;
+    auto F = [&](int field) -> double& { return f[(size_t)field * items + tt]; };
+    if (round == 0 && !A.rates_ready) {
+        const long long e = tt - (long long)m * n;
+        double s0 = 0, s1 = 0, s2 = 0;
+        for (int c = 0; c < A.nchunk; ++c) {
+            s0 += A.partial[(((size_t)c * 3 + 0) * B + m) * A.npad + e];
+            s1 += A.partial[(((size_t)c * 3 + 1) * B + m) * A.npad + e];
+            s2 += A.partial[(((size_t)c * 3 + 2) * B + m) * A.npad + e];
+        }
+        F(E_ADOTV1) = s0;
+        F(E_ADOTV2) = s1;
+        F(E_DSIGMADT) = A.fix_tn ? F(E_NORMAL_LOADING) : s2 + F(E_NORMAL_LOADING);
+    }
+    EElem e;
+    e.a = F(E_A); e.b = F(E_B); e.dc = F(E_DC); e.f0 = F(E_F0);
+    e.P = F(E_P); e.dPdt = F(E_DPDT0); e.sl = F(E_SHEAR_LOADING);
+    e.A1 = F(E_ADOTV1); e.A2 = F(E_ADOTV2); e.dsig = F(E_DSIGMADT);
+    e.V = F(E_SLIPV);
+    elem_constants(e, A.V0);
+    const double y[4] = {F(E_TNO), F(E_TT1O), F(E_TT2O), F(E_STATE)};
+    double y5[4], y4[4];
+    ens_dp45(e, A.V0, A.eta, y, mc[m].h, y5, y4);
+    F(E_Y5_0) = y5[0]; F(E_Y5_1) = y5[1]; F(E_Y5_2) = y5[2]; F(E_Y5_3) = y5[3];
+    const double r1 = fabs(fabs(y4[3] - y5[3]) / y4[3]);
+    const double rt1 = fabs(fabs(y4[1] - y5[1]) / y4[1]);
+    const double rt2 = fabs(fabs(y4[2] - y5[2]) / y4[2]);
+    const double r2 = (rt2 > rt1 || rt2 != rt2) ? rt2 : rt1;
+    r1b = ebits(r1);
+    r2b = ebits(r2);
+}
+
 // One attempt for every item of a pending member; round 0 first sums the
-// GEMM's split-K partials into the frozen traction rates.
+// GEMM's split-K partials into the frozen traction rates (unless
+// ens_rates_kernel has).  The first round (FIRST, round 0) runs one item
+// per thread over all items; the later rounds (grid ENS_LATER_BLOCKS) stride
+// over the pending members' items only, so a round with few or no pending
+// members costs little.
+template <bool FIRST>
 __global__ __launch_bounds__(256, 5) void ens_attempt_kernel(EnsArgs A, int round) {
     const long long n = A.n;
     const int B = A.B;
     const long long items = n * B;
     const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     const MemberCtrl* mc = A.mc + (size_t)(round & 1) * B;
+    unsigned long long* slot = A.slot + (size_t)round * B * 2;
     // the previous round's reduction slot is free again (its readers have finished)
     const int prev = (round + ENS_ROUNDS - 1) % ENS_ROUNDS;
     if (t < 2 * B) A.slot[(size_t)prev * B * 2 + t] = 0ull;
-    const bool in = t < items;
-    const long long tt = in ? t : items - 1;
-    const int m = (int)(tt / n);
-    const bool pend = in && pending(mc[m]);
-    double* f = A.f;
-    auto F = [&](int field) -> double& { return f[(size_t)field * items + tt]; };
-    unsigned long long r1b = 0ull, r2b = 0ull;
-    if (pend) {
-        if (round == 0 && !A.rates_ready) {
-            const long long e = tt - (long long)m * n;
-            double s0 = 0, s1 = 0, s2 = 0;
-            for (int c = 0; c < A.nchunk; ++c) {
-                s0 += A.partial[(((size_t)c * 3 + 0) * B + m) * A.npad + e];
-                s1 += A.partial[(((size_t)c * 3 + 1) * B + m) * A.npad + e];
-                s2 += A.partial[(((size_t)c * 3 + 2) * B + m) * A.npad + e];
-            }
-            F(E_ADOTV1) = s0;
-            F(E_ADOTV2) = s1;
-            F(E_DSIGMADT) = A.fix_tn ? F(E_NORMAL_LOADING) : s2 + F(E_NORMAL_LOADING);
-        }
-        EElem e;
-        e.a = F(E_A); e.b = F(E_B); e.dc = F(E_DC); e.f0 = F(E_F0);
-        e.P = F(E_P); e.dPdt = F(E_DPDT0); e.sl = F(E_SHEAR_LOADING);
-        e.A1 = F(E_ADOTV1); e.A2 = F(E_ADOTV2); e.dsig = F(E_DSIGMADT);
-        e.V = F(E_SLIPV);
-        elem_constants(e, A.V0);
-        const double y[4] = {F(E_TNO), F(E_TT1O), F(E_TT2O), F(E_STATE)};
-        double y5[4], y4[4];
-        ens_dp45(e, A.V0, A.eta, y, mc[m].h, y5, y4);
-        F(E_Y5_0) = y5[0]; F(E_Y5_1) = y5[1]; F(E_Y5_2) = y5[2]; F(E_Y5_3) = y5[3];
-        const double r1 = fabs(fabs(y4[3] - y5[3]) / y4[3]);
-        const double rt1 = fabs(fabs(y4[1] - y5[1]) / y4[1]);
-        const double rt2 = fabs(fabs(y4[2] - y5[2]) / y4[2]);
-        const double r2 = (rt2 > rt1 || rt2 != rt2) ? rt2 : rt1;
-        r1b = ebits(r1);
-        r2b = ebits(r2);
+    if (FIRST) {
+        const bool in = t < items;
+        const long long tt = in ? t : items - 1;
+        const int m = (int)(tt / n);
+        const bool pend = in && pending(mc[m]);
+        unsigned long long r1b = 0ull, r2b = 0ull;
+        if (pend) attempt_item(A, round, mc, tt, m, r1b, r2b);
+        member_max_atomic(m, pend, r1b, r2b, slot);
+        return;
     }
-    member_max_atomic(m, pend, r1b, r2b, A.slot + (size_t)round * B * 2);
+    __shared__ int s_list[64];
+    __shared__ int s_count;
+    member_list(threadIdx.x < (unsigned)B && pending(mc[threadIdx.x < (unsigned)B ? threadIdx.x : 0]), s_list,
+                &s_count);
+    __syncthreads();
+    const long long total = (long long)s_count * n;  // the pending members' items, concatenated
+    const long long stride = (long long)gridDim.x * blockDim.x;
+    // the loop bound is wave-uniform: member_max_atomic shuffles across the wave
+    for (long long b = (long long)blockIdx.x * blockDim.x + (threadIdx.x & ~63u); b < total; b += stride) {
+        const long long k = b + (threadIdx.x & 63u);
+        const bool valid = k < total;
+        const long long kc = valid ? k : total - 1;
+        const int j = (int)(kc / n);
+        const int m = s_list[j];
+        const long long tt = (long long)m * n + (kc - (long long)j * n);
+        unsigned long long r1b = 0ull, r2b = 0ull;
+        if (valid) attempt_item(A, round, mc, tt, m, r1b, r2b);
+        member_max_atomic(m, valid, r1b, r2b, slot);
+    }
+}
+
+// Post-step update of item t after an accepted step of size h
+// (QDsim_gpu.py:986-1022): the candidate state becomes the state; slip
+// rates, traction, rake, slip and friction follow from it.
+__device__ __forceinline__ void post_step_item(const EnsArgs& A, long long t, double h) {
+    const long long items = A.n * A.B;
+    double* f = A.f;
+    auto F = [&](int field) -> double& { return f[(size_t)field * items + t]; };
+    const double Tno = F(E_Y5_0), Tt1 = F(E_Y5_1), Tt2 = F(E_Y5_2), st = F(E_Y5_3);
+    const double a = F(E_A);
+    F(E_TNO) = Tno; F(E_TT1O) = Tt1; F(E_TT2O) = Tt2; F(E_STATE) = st;
+    const double Tt = sqrt(Tt1 * Tt1 + Tt2 * Tt2);
+    F(E_TT) = Tt;
+    const double ex = 2.0 * A.V0 * exp(-st / a);
+    const double v1 = ex * sinh(Tt1 / Tno / a), v2 = ex * sinh(Tt2 / Tno / a);
+    F(E_SLIPV1) = v1; F(E_SLIPV2) = v2;
+    double v = sqrt(v1 * v1 + v2 * v2);
+    F(E_RAKE) = atan2(v2, v1);
+    if (v < 1e-30) v = 1e-30;
+    F(E_SLIPV) = v;
+    const double s1 = F(E_SLIP1) + v1 * h, s2 = F(E_SLIP2) + v2 * h;
+    F(E_SLIP1) = s1; F(E_SLIP2) = s2; F(E_SLIP) = sqrt(s1 * s1 + s2 * s2);
+    F(E_FRIC) = Tt / (Tno - F(E_P) * 1e-6);
 }
 
 // Accept/reject per member (replicated in every workgroup) and post-step update
 // of the items of members accepted in this round (QDsim_gpu.py:959-1022).
+// The first round (FIRST, round 0) runs one item per thread; the later
+// rounds (grid ENS_LATER_BLOCKS) stride over the items of the members
+// accepted in them.
+template <bool FIRST>
 __global__ __launch_bounds__(256) void ens_decide_kernel(EnsArgs A, int round) {
     const long long n = A.n;
     const int B = A.B;
@@ -811,27 +886,23 @@ __global__ __launch_bounds__(256) void ens_decide_kernel(EnsArgs A, int round) {
     }
     __syncthreads();
     const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t < items) {
-        const int m = (int)(t / n);
-        if (s_acc[m]) {
-            double* f = A.f;
-            auto F = [&](int field) -> double& { return f[(size_t)field * items + t]; };
-            const double h = s_mc[m].last_h;
-            const double Tno = F(E_Y5_0), Tt1 = F(E_Y5_1), Tt2 = F(E_Y5_2), st = F(E_Y5_3);
-            const double a = F(E_A);
-            F(E_TNO) = Tno; F(E_TT1O) = Tt1; F(E_TT2O) = Tt2; F(E_STATE) = st;
-            const double Tt = sqrt(Tt1 * Tt1 + Tt2 * Tt2);
-            F(E_TT) = Tt;
-            const double ex = 2.0 * A.V0 * exp(-st / a);
-            const double v1 = ex * sinh(Tt1 / Tno / a), v2 = ex * sinh(Tt2 / Tno / a);
-            F(E_SLIPV1) = v1; F(E_SLIPV2) = v2;
-            double v = sqrt(v1 * v1 + v2 * v2);
-            F(E_RAKE) = atan2(v2, v1);
-            if (v < 1e-30) v = 1e-30;
-            F(E_SLIPV) = v;
-            const double s1 = F(E_SLIP1) + v1 * h, s2 = F(E_SLIP2) + v2 * h;
-            F(E_SLIP1) = s1; F(E_SLIP2) = s2; F(E_SLIP) = sqrt(s1 * s1 + s2 * s2);
-            F(E_FRIC) = Tt / (Tno - F(E_P) * 1e-6);
+    if (FIRST) {
+        if (t < items) {
+            const int m = (int)(t / n);
+            if (s_acc[m]) post_step_item(A, t, s_mc[m].last_h);
+        }
+    } else {
+        // the items of the members accepted in this round only
+        __shared__ int s_list[64];
+        __shared__ int s_count;
+        member_list(threadIdx.x < (unsigned)B && s_acc[threadIdx.x < (unsigned)B ? threadIdx.x : 0] != 0, s_list,
+                    &s_count);
+        __syncthreads();
+        const long long total = (long long)s_count * n;
+        for (long long k = t; k < total; k += (long long)gridDim.x * blockDim.x) {
+            const int j = (int)(k / n);
+            const int m = s_list[j];
+            post_step_item(A, (long long)m * n + (k - (long long)j * n), s_mc[m].last_h);
         }
     }
     if (blockIdx.x == 0 && threadIdx.x < B) {
@@ -1001,9 +1072,12 @@ static void ens_launch_step(Ensemble* e, Launch& L) {
     const unsigned blocks = (unsigned)((e->n * e->B + 255) / 256);
     L(ens_rates_kernel, dim3(blocks), dim3(256), a);
     a.rates_ready = 1;
-    for (int r = 0; r < ENS_ROUNDS; ++r) {
-        L(ens_attempt_kernel, dim3(blocks), dim3(256), a, r);
-        L(ens_decide_kernel, dim3(blocks), dim3(256), a, r);
+    const unsigned later = blocks < (unsigned)ENS_LATER_BLOCKS ? blocks : (unsigned)ENS_LATER_BLOCKS;
+    L(ens_attempt_kernel<true>, dim3(blocks), dim3(256), a, 0);
+    L(ens_decide_kernel<true>, dim3(blocks), dim3(256), a, 0);
+    for (int r = 1; r < ENS_ROUNDS; ++r) {
+        L(ens_attempt_kernel<false>, dim3(later), dim3(256), a, r);
+        L(ens_decide_kernel<false>, dim3(later), dim3(256), a, r);
     }
 }
 
