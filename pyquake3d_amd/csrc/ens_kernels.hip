@@ -16,21 +16,23 @@
 //      source-major matrices (row = 4*(l%16) + subtile) and the four 16-member
 //      subtiles from one 16-byte load of u (member = 4*(l%16) + subtile).
 //      Split-K partials per column chunk are summed in a fixed order.
-//      ens_gemm3_kernel (precision mode 1, when Bs = Bd = 0) is the same tile
-//      with every FP32 operand split exactly into three BF16 parts and the
-//      six leading part products on v_mfma_f32_16x16x32_bf16.
+//      ens_gemm3_kernel (precision mode 1) is the same tile with every FP32
+//      operand split exactly into three BF16 parts and the six leading part
+//      products on v_mfma_f32_16x16x32_bf16 (a shear pass, and a normal pass
+//      over Bs, Bd when they are nonzero).
 //   3. ens_rates_kernel:   the partials summed in chunk order into the active
-//      members' frozen traction rates.
-//   4. R rounds of
-//        ens_attempt_kernel: one Dormand-Prince attempt for every (member,
-//          element) item of members still pending in this step; per-member
-//          error maxima by wave shuffles and one atomic per wave;
-//        ens_decide_kernel:  every workgroup applies the reference's
-//          accept/reject rule (QDsim_gpu.py:959-985) to the same reduced
-//          errors; items of members accepted in this round write the
-//          post-step state; workgroup 0 publishes the member control into the
-//          other half of a double buffer (readers of this launch see the old
-//          half).
+//      members' frozen traction rates; every round's error slots zeroed.
+//   4. R = ENS_ROUNDS attempt rounds:
+//        ens_attempt_kernel: round 0, one Dormand-Prince attempt for every
+//          (member, element) item of an active member; per-member error
+//          maxima by wave shuffles and one atomic per wave;
+//        ens_decide_kernel (R launches): every workgroup applies the
+//          reference's accept/reject rule (QDsim_gpu.py:959-985) to the same
+//          reduced errors of round r; items of members accepted in it write
+//          the post-step state; the members it rejected make round r + 1's
+//          attempts in the same launch; workgroup 0 publishes the member
+//          control into the other half of a double buffer (readers of this
+//          launch see the old half).
 //      A member still rejecting after R rounds keeps its state, reduced step
 //      size and rejection count and continues at the next global step, whose
 //      GEMM reproduces the same frozen traction rates (the reference freezes
@@ -681,6 +683,9 @@ __device__ __forceinline__ bool pending(const MemberCtrl& c) { return c.active &
 __global__ __launch_bounds__(256) void ens_rates_kernel(EnsArgs A) {
     const long long n = A.n, items = n * A.B;
     const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    // every round's error-maximum slots start the step at zero (their last
+    // readers, the previous step's decisions, have finished)
+    if (t < (long long)ENS_ROUNDS * 2 * A.B) A.slot[t] = 0ull;
     if (t >= items) return;
     const int m = (int)(t / n);
     if (!A.mc[m].active) return;  // round 0 reads A.mc's first half, as here
@@ -764,52 +769,22 @@ __device__ __forceinline__ void attempt_item(const EnsArgs& A, int round, const 
     r2b = ebits(r2);
 }
 
-// One attempt for every item of a pending member; round 0 first sums the
-// GEMM's split-K partials into the frozen traction rates (unless
-// ens_rates_kernel has).  The first round (FIRST, round 0) runs one item
-// per thread over all items; the later rounds (grid ENS_LATER_BLOCKS) stride
-// over the pending members' items only, so a round with few or no pending
-// members costs little.
-template <bool FIRST>
-__global__ __launch_bounds__(256, 5) void ens_attempt_kernel(EnsArgs A, int round) {
+// Round 0: one attempt for every item of a pending member, one item per
+// thread; it first sums the GEMM's split-K partials into the frozen traction
+// rates unless ens_rates_kernel has.  (The later rounds' attempts run inside
+// ens_decide_kernel, after the previous round's decisions.)
+__global__ __launch_bounds__(256, 5) void ens_attempt_kernel(EnsArgs A) {
     const long long n = A.n;
-    const int B = A.B;
-    const long long items = n * B;
+    const long long items = n * A.B;
     const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    const MemberCtrl* mc = A.mc + (size_t)(round & 1) * B;
-    unsigned long long* slot = A.slot + (size_t)round * B * 2;
-    // the previous round's reduction slot is free again (its readers have finished)
-    const int prev = (round + ENS_ROUNDS - 1) % ENS_ROUNDS;
-    if (t < 2 * B) A.slot[(size_t)prev * B * 2 + t] = 0ull;
-    if (FIRST) {
-        const bool in = t < items;
-        const long long tt = in ? t : items - 1;
-        const int m = (int)(tt / n);
-        const bool pend = in && pending(mc[m]);
-        unsigned long long r1b = 0ull, r2b = 0ull;
-        if (pend) attempt_item(A, round, mc, tt, m, r1b, r2b);
-        member_max_atomic(m, pend, r1b, r2b, slot);
-        return;
-    }
-    __shared__ int s_list[64];
-    __shared__ int s_count;
-    member_list(threadIdx.x < (unsigned)B && pending(mc[threadIdx.x < (unsigned)B ? threadIdx.x : 0]), s_list,
-                &s_count);
-    __syncthreads();
-    const long long total = (long long)s_count * n;  // the pending members' items, concatenated
-    const long long stride = (long long)gridDim.x * blockDim.x;
-    // the loop bound is wave-uniform: member_max_atomic shuffles across the wave
-    for (long long b = (long long)blockIdx.x * blockDim.x + (threadIdx.x & ~63u); b < total; b += stride) {
-        const long long k = b + (threadIdx.x & 63u);
-        const bool valid = k < total;
-        const long long kc = valid ? k : total - 1;
-        const int j = (int)(kc / n);
-        const int m = s_list[j];
-        const long long tt = (long long)m * n + (kc - (long long)j * n);
-        unsigned long long r1b = 0ull, r2b = 0ull;
-        if (valid) attempt_item(A, round, mc, tt, m, r1b, r2b);
-        member_max_atomic(m, valid, r1b, r2b, slot);
-    }
+    const MemberCtrl* mc = A.mc;
+    const bool in = t < items;
+    const long long tt = in ? t : items - 1;
+    const int m = (int)(tt / n);
+    const bool pend = in && pending(mc[m]);
+    unsigned long long r1b = 0ull, r2b = 0ull;
+    if (pend) attempt_item(A, 0, mc, tt, m, r1b, r2b);
+    member_max_atomic(m, pend, r1b, r2b, A.slot);
 }
 
 // Post-step update of item t after an accepted step of size h
@@ -836,13 +811,17 @@ __device__ __forceinline__ void post_step_item(const EnsArgs& A, long long t, do
     F(E_FRIC) = Tt / (Tno - F(E_P) * 1e-6);
 }
 
-// Accept/reject per member (replicated in every workgroup) and post-step update
-// of the items of members accepted in this round (QDsim_gpu.py:959-1022).
-// The first round (FIRST, round 0) runs one item per thread; the later
-// rounds (grid ENS_LATER_BLOCKS) stride over the items of the members
-// accepted in them.
-template <bool FIRST>
-__global__ __launch_bounds__(256) void ens_decide_kernel(EnsArgs A, int round) {
+// Round `round`'s accept/reject per member (replicated in every workgroup)
+// and post-step update of the items of members accepted in it
+// (QDsim_gpu.py:959-1022); then, with NEXT, round + 1's attempts of the
+// members still pending, from the decisions this workgroup holds.  The two
+// parts touch disjoint items (accepted and rejected members).  FIRST (round
+// 0, full grid): the update runs one item per thread; otherwise (grid
+// ENS_LATER_BLOCKS) it strides over the accepted members' items.  The
+// attempts always stride over the pending members' items only, so a round
+// with few or no pending members costs little.
+template <bool FIRST, bool NEXT>
+__global__ __launch_bounds__(256, 5) void ens_decide_kernel(EnsArgs A, int round) {
     const long long n = A.n;
     const int B = A.B;
     const long long items = n * B;
@@ -903,6 +882,30 @@ __global__ __launch_bounds__(256) void ens_decide_kernel(EnsArgs A, int round) {
             const int j = (int)(k / n);
             const int m = s_list[j];
             post_step_item(A, (long long)m * n + (k - (long long)j * n), s_mc[m].last_h);
+        }
+    }
+    if (NEXT) {
+        // round + 1: the members this round rejected (still active), with
+        // the step sizes just decided
+        __shared__ int s_pend[64];
+        __shared__ int s_npend;
+        member_list(threadIdx.x < (unsigned)B && pending(s_mc[threadIdx.x < (unsigned)B ? threadIdx.x : 0]),
+                    s_pend, &s_npend);
+        __syncthreads();
+        unsigned long long* slot = A.slot + (size_t)(round + 1) * B * 2;
+        const long long total = (long long)s_npend * n;  // their items, concatenated
+        const long long stride = (long long)gridDim.x * blockDim.x;
+        // the loop bound is wave-uniform: member_max_atomic shuffles across the wave
+        for (long long b = (long long)blockIdx.x * blockDim.x + (threadIdx.x & ~63u); b < total; b += stride) {
+            const long long k = b + (threadIdx.x & 63u);
+            const bool valid = k < total;
+            const long long kc = valid ? k : total - 1;
+            const int j = (int)(kc / n);
+            const int m = s_pend[j];
+            const long long tt = (long long)m * n + (kc - (long long)j * n);
+            unsigned long long r1b = 0ull, r2b = 0ull;
+            if (valid) attempt_item(A, round + 1, s_mc, tt, m, r1b, r2b);
+            member_max_atomic(m, valid, r1b, r2b, slot);
         }
     }
     if (blockIdx.x == 0 && threadIdx.x < B) {
@@ -1073,12 +1076,10 @@ static void ens_launch_step(Ensemble* e, Launch& L) {
     L(ens_rates_kernel, dim3(blocks), dim3(256), a);
     a.rates_ready = 1;
     const unsigned later = blocks < (unsigned)ENS_LATER_BLOCKS ? blocks : (unsigned)ENS_LATER_BLOCKS;
-    L(ens_attempt_kernel<true>, dim3(blocks), dim3(256), a, 0);
-    L(ens_decide_kernel<true>, dim3(blocks), dim3(256), a, 0);
-    for (int r = 1; r < ENS_ROUNDS; ++r) {
-        L(ens_attempt_kernel<false>, dim3(later), dim3(256), a, r);
-        L(ens_decide_kernel<false>, dim3(later), dim3(256), a, r);
-    }
+    L(ens_attempt_kernel, dim3(blocks), dim3(256), a);
+    L(ens_decide_kernel<true, true>, dim3(blocks), dim3(256), a, 0);
+    for (int r = 1; r < ENS_ROUNDS - 1; ++r) L(ens_decide_kernel<false, true>, dim3(later), dim3(256), a, r);
+    L(ens_decide_kernel<false, false>, dim3(later), dim3(256), a, ENS_ROUNDS - 1);
 }
 
 }  // namespace pq3d
