@@ -213,7 +213,7 @@ int pq3d_dense_dilatancy_update(void* sim, double dt);
  *    the reference runs independent QDsim_gpu instances)
  *
  * One global step = one tall-skinny MFMA GEMM streaming the six FP32 matrices
- * once for all members, then up to two Dormand-Prince attempt rounds per
+ * once for all members, then up to four Dormand-Prince attempt rounds per
  * active member with its own step size (a member still rejecting continues
  * at the next global step with the same frozen traction rates).  All work of
  * an ensemble runs on its own stream.  Field arrays are B x n (member-major),
