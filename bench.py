@@ -634,8 +634,8 @@ def main():
                     help="only copy a device buffer of this size 5 times (calibrates PMC byte counters)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
-        print("bench.py: this build runs on one GPU only; how the path would shard is described in DESIGN.md",
-              file=sys.stderr)
+        print("bench.py: one GPU only; runs on more than one GPU are not supported (members, row panels or "
+              "H-matrix row clusters would be sharded across GPUs, which this code does not do)", file=sys.stderr)
         sys.exit(2)
 
     if args.copy_probe_gib > 0:
