@@ -37,7 +37,51 @@ PQ_HD double rsqrt_(double a) { return sqrt(a); }
 PQ_HD double rlog_(double a) { return log(a); }
 PQ_HD double ratan_(double a) { return atan(a); }
 PQ_HD double ratan2_(double a, double b) { return atan2(a, b); }
-PQ_HD double rrcp_(double a) { return 1.0 / a; }
+// 1 / a.  On the device: the hardware estimate refined by two Newton steps
+// (within an ulp of the quotient, without the IEEE division's scaling and
+// fix-up).  At a = 0 or a = inf it gives NaN where the division gives inf or
+// 0; only singular receivers (on a dislocation line or vertex) reach those,
+// and their outputs are not finite either way.
+PQ_HD double rrcp_(double a) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    double r = __builtin_amdgcn_rcp(a);
+    double e = fma(-a, r, 1.0);
+    r = fma(r, e, r);
+    e = fma(-a, r, 1.0);
+    return fma(r, e, r);
+#else
+    return 1.0 / a;
+#endif
+}
+
+// s = sqrt(a) and h = 0.5 / s together.  Generic form: the primitives above
+// (so a counting type counts a square root and a reciprocal).  On the device
+// for double: the hardware reciprocal square root refined by two coupled
+// Newton (Goldschmidt) steps for g -> sqrt(a) and h -> 1 / (2 sqrt(a)), then
+// one residual correction of g; a > 0 (a = 0 only at a singular receiver).
+template <class R>
+PQ_HD void sqrt_hrcp(const R& a, R& s, R& h) {
+    s = rsqrt_(a);
+    h = R(0.5) * rrcp_(s);
+}
+PQ_HD void sqrt_hrcp(const double& a, double& s, double& h) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    const double y = __builtin_amdgcn_rsq(a);
+    double g = a * y, hh = 0.5 * y;
+    double r = fma(-g, hh, 0.5);
+    g = fma(g, r, g);
+    hh = fma(hh, r, hh);
+    r = fma(-g, hh, 0.5);
+    g = fma(g, r, g);
+    hh = fma(hh, r, hh);
+    const double d = fma(-g, g, a);
+    s = fma(d, hh, g);
+    h = hh;
+#else
+    s = sqrt(a);
+    h = 0.5 / s;
+#endif
+}
 
 // --------------------------------------------------------------------------
 // Forward-mode dual number: value and gradient with respect to (X, Y, Z)
@@ -66,8 +110,8 @@ template <class R> PQ_HD Dual<R> rcp(const Dual<R>& a) {
     return {q, a.x * m, a.y * m, a.z * m};
 }
 template <class R> PQ_HD Dual<R> dsqrt(const Dual<R>& a) {
-    R s = rsqrt_(a.v);
-    R h = R(0.5) * rrcp_(s);
+    R s, h;
+    sqrt_hrcp(a.v, s, h);
     return {s, a.x * h, a.y * h, a.z * h};
 }
 template <class R> PQ_HD Dual<R> dlog(const Dual<R>& a) {
