@@ -101,7 +101,12 @@ int pq3d_probe_fma64(int iters, int reps, float* ms_per, double* tflops);
  *   rows, cols : optional index lists (nr, nc entries), NULL = all
  *   d_out[6]   : device buffers; element (i, j) of the nr x nc block at
  *                j*ld + i (source-major), out_type PQ3D_MAT_F64 or _F32
- *   src_per_block: sources walked per workgroup (<= 0: default) */
+ *   src_per_block: consecutive sources (columns) walked per one-wave
+ *                workgroup (<= 0: the longest run up to 64 that still gives
+ *                8,192 workgroups).  Within a run, the free-surface
+ *                correction of a side shared by consecutive sources is
+ *                evaluated once for both; entries agree with the
+ *                every-side evaluation to rounding (a few 1e-11 relative). */
 int pq3d_gf_assemble(const double* P1, const double* P2, const double* P3, long long nsrc,
                      const double* xg, const double* ev, long long nrcv, double mu, double lam,
                      int halfspace, int out_type, void* const* d_out, long long ld,

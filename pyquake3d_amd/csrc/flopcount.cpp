@@ -8,7 +8,10 @@
 //
 // Usage: flopcount [nx] [samples]  -> prints the mean count per pair for the
 // planar nx x nx mesh of a 40 km x 40 km vertical fault (strike and dip slip,
-// half space, traction projection included), as assembled by gf_assemble_kernel.
+// half space, traction projection included), as assembled by gf_assemble_kernel:
+// runs of 64 consecutive sources per receiver, each side shared by consecutive
+// sources evaluated once (first line), and the count with every side
+// evaluated for every pair (second line).
 #include <cstdio>
 #include <cstdlib>
 #include <random>
@@ -75,27 +78,64 @@ int main(int argc, char** argv) {
     const long long N = (long long)tri.size() / 9;
     std::mt19937_64 rng(12345);
     std::uniform_int_distribution<long long> pick(0, N - 1);
+    const int RUN = 64;  // sources per workgroup run (the library's default at this size)
+    std::uniform_int_distribution<long long> pick_run(0, N / RUN - 1);
     const double bT[2][3] = {{0.0, 1.0, 0.0}, {0.0, 0.0, 1.0}};
-    long long total = 0;
-    for (int s = 0; s < samples; ++s) {
-        const long long r = pick(rng), j = pick(rng);
-        const double* R = &tri[9 * r];
-        double X = (R[0] + R[3] + R[6]) / 3, Y = (R[1] + R[4] + R[7]) / 3, Z = (R[2] + R[5] + R[8]) / 3;
-        SrcGeom<2> g;
-        src_setup<2>(&tri[9 * j], &tri[9 * j + 3], &tri[9 * j + 6], bT, g);
-        // receiver frame of a y = 0 plane with +y normal: e1 = (1,0,0), e2 = (0,0,-1), n = (0,1,0)
-        const Cnt ev[9] = {1, 0, 0, 0, 0, -1, 0, 1, 0};
-        g_flops = 0;
-        Cnt G[2][9];
-        td_pair_grad<Cnt, 2, false>(g, Cnt(X), Cnt(Y), Cnt(Z), Cnt(nu), true, G);
+    // receiver frame of a y = 0 plane with +y normal: e1 = (1,0,0), e2 = (0,0,-1), n = (0,1,0)
+    const Cnt ev[9] = {1, 0, 0, 0, 0, -1, 0, 1, 0};
+    auto tri_at = [&](long long j, double (*V)[3]) {
+        for (int v = 0; v < 3; ++v)
+            for (int q = 0; q < 3; ++q) V[v][q] = tri[9 * j + 3 * v + q];
+    };
+    auto project = [&](Cnt (*G)[9]) {
         for (int b = 0; b < 2; ++b) {
             Cnt eps[6], sig[6], t[3];
             grad_to_strain_stress<Cnt>(G[b], Cnt(mu), Cnt(lam), eps, sig);
             stress_to_traction<Cnt>(sig, ev, t);
         }
-        total += g_flops;
+    };
+    long long shared_total = 0, plain_total = 0, pairs = 0;
+    const int runs = samples / RUN > 0 ? samples / RUN : 1;
+    for (int s = 0; s < runs; ++s) {
+        const long long r = pick(rng), j0 = pick_run(rng) * RUN;
+        const double* R = &tri[9 * r];
+        double X = (R[0] + R[3] + R[6]) / 3, Y = (R[1] + R[4] + R[7]) / 3, Z = (R[2] + R[5] + R[8]) / 3;
+        Cnt carry[2][3][9];
+        for (long long j = j0; j < j0 + RUN; ++j) {
+            SrcGeom<2> g;
+            src_setup<2>(&tri[9 * j], &tri[9 * j + 3], &tri[9 * j + 6], bT, g);
+            double V[3][3], W[3][3];
+            tri_at(j, V);
+            if (j > 0) {
+                tri_at(j - 1, W);
+                g.prev_side = shared_side(V, W, &g.prev_rev);
+            }
+            if (j + 1 < N) {
+                int rev;
+                tri_at(j + 1, W);
+                g.next_side = shared_side(V, W, &rev);
+            }
+            if (g.prev_side >= 0 && g.prev_side == g.next_side) g.prev_side = -1;
+            const int cur = (int)((j - j0) & 1);
+            Cnt G[2][9];
+            g_flops = 0;
+            td_pair_grad_shared<Cnt>(
+                g, Cnt(X), Cnt(Y), Cnt(Z), Cnt(nu), true, j > j0,
+                [&](int k, int c, Cnt v) { carry[cur][k][c] = v; }, [&](int k, int c) { return carry[cur ^ 1][k][c]; },
+                G);
+            project(G);
+            shared_total += g_flops;
+            g_flops = 0;
+            td_pair_grad<Cnt, 2, false>(g, Cnt(X), Cnt(Y), Cnt(Z), Cnt(nu), true, G);
+            project(G);
+            plain_total += g_flops;
+            ++pairs;
+        }
     }
-    printf("%.1f flops_per_pair (mean of %d pairs, %dx%d planar mesh, half space, strike+dip, "
-           "add/sub/mul/div/sqrt/log/atan = 1)\n", (double)total / samples, samples, nx, nx);
+    printf("%.1f flops_per_pair (mean of %lld pairs in runs of %d consecutive sources, %dx%d planar mesh, half "
+           "space, strike+dip, add/sub/mul/div/sqrt/log/atan = 1, sides shared by consecutive sources evaluated "
+           "once)\n", (double)shared_total / pairs, pairs, RUN, nx, nx);
+    printf("%.1f flops_per_pair_every_side (the same pairs with every side evaluated for every pair)\n",
+           (double)plain_total / pairs);
     return 0;
 }

@@ -158,7 +158,10 @@ struct SrcGeom {
     double bT[NB][3];          // TDCS Burgers vectors (bx = Ts, by = Ss, bz = Ds)
     double bS[3][NB][3];       // per FSC side: EFCS Burgers vector in the side frame
     int flush;                 // all three vertices at z = 0
-    int pad;
+    // Sides shared with the neighbouring table entries (-1: none): the side
+    // shared with the previous entry, whether the previous entry traverses it
+    // the other way, and the side shared with the next entry.
+    int prev_side, prev_rev, next_side;
 };
 
 PQ_HD void cross3(const double* a, const double* b, double* c) {
@@ -268,7 +271,8 @@ PQ_HD void src_setup(const double* P1, const double* P2, const double* P3, const
     fsc_side_setup(P2, P3, g.side[1]);
     fsc_side_setup(P3, P1, g.side[2]);
     g.flush = (P1[2] == 0.0 && P2[2] == 0.0 && P3[2] == 0.0) ? 1 : 0;
-    g.pad = 0;
+    g.prev_side = g.next_side = -1;
+    g.prev_rev = 0;
     for (int b = 0; b < NB; ++b) {
         double Bv[3];
         for (int k = 0; k < 3; ++k) {
@@ -447,12 +451,41 @@ PQ_HD void fsc_scatter(const double (*bS)[3], int k, R K, R e1x, R e1y, const Du
     }
 }
 
+// G[k] += K * grad(v1 ey1 + v2 ey2 + v3 ey3): the response to the unit
+// Burgers vector along side-frame axis k (BASIS mode of the side evaluation)
+template <class R>
+PQ_HD void fsc_scatter_basis(int k, R K, R e1x, R e1y, const Dual<R>& v1, const Dual<R>& v2, const Dual<R>& v3,
+                             R (*G)[9]) {
+    const R a1 = K * e1x, a2 = K * e1y;
+    G[k][0] += a1 * v1.x + a2 * v2.x;
+    G[k][1] += a1 * v1.y + a2 * v2.y;
+    G[k][2] += a1 * v1.z + a2 * v2.z;
+    G[k][3] += a2 * v1.x - a1 * v2.x;
+    G[k][4] += a2 * v1.y - a1 * v2.y;
+    G[k][5] += a2 * v1.z - a1 * v2.z;
+    G[k][6] -= K * v3.x;
+    G[k][7] -= K * v3.y;
+    G[k][8] -= K * v3.z;
+}
+
+template <class R, int NB, bool BASIS>
+PQ_HD void fsc_emit(const double (*bS)[3], int k, R K, R e1x, R e1y, const Dual<R>& v1, const Dual<R>& v2,
+                    const Dual<R>& v3, R (*G)[9]) {
+    if (BASIS)
+        fsc_scatter_basis<R>(k, K, e1x, e1y, v1, v2, v3, G);
+    else
+        fsc_scatter<R, NB>(bS, k, K, e1x, e1y, v1, v2, v3, G);
+}
+
 // --------------------------------------------------------------------------
 // Free-surface correction: one end of one side (AngDisDispFSC,
 // DH_greenfunction.py:273-351), accumulated as sgn * grad u into G.
 // cs = {sinB, cosB, cotB, 1/tan(B/2)} of the configuration in use.
+// BASIS: G has three rows, row k the response to the unit Burgers vector
+// along side-frame axis k (bS unused); otherwise G[b] is the response to
+// Burgers vector bS[b] (side-frame components).
 // --------------------------------------------------------------------------
-template <class R, int NB>
+template <class R, int NB, bool BASIS = false>
 PQ_HD void fsc_end_grad(const Dual<R>& y1, const Dual<R>& y2, const Dual<R>& y3, const double* cs, R a,
                         R nu, const double (*bS)[3], R e1x, R e1y, R sgn, R (*G)[9]) {
     const R s = R(cs[0]), c = R(cs[1]), ct = R(cs[2]), ith = R(cs[3]);
@@ -507,7 +540,7 @@ PQ_HD void fsc_end_grad(const Dual<R>& y1, const Dual<R>& y2, const Dual<R>& y3,
             (R(2) * k2) * ((k1 * ct) * Fib + y2 * iry * n2arb - (y2 * c) * irz * carb) +
             y2W * irb * ((R(2) * n) * iry + a * ir2) +
             (y2W * c) * irbirz * (k1 - rc * irz * carb - a * y3b * ir2);
-        fsc_scatter<R, NB>(bS, 0, K, e1x, e1y, v1, v2, v3, G);
+        fsc_emit<R, NB, BASIS>(bS, 0, K, e1x, e1y, v1, v2, v3, G);
     }
     // ---- b2 basis
     {
@@ -533,7 +566,7 @@ PQ_HD void fsc_end_grad(const Dual<R>& y1, const Dual<R>& y2, const Dual<R>& y3,
             W * irb * (k1 * ct - (R(2) * n) * y1 * iry - a * y1 * ir2) -
             W * irz * (c * s + ct * rc * irb * ((R(2) * k2 * c) - rc * irz) +
                        arb * (s - y3b * z1b * ir2 - z1b * rc * irbirz));
-        fsc_scatter<R, NB>(bS, 1, K, e1x, e1y, v1, v2, v3, G);
+        fsc_emit<R, NB, BASIS>(bS, 1, K, e1x, e1y, v1, v2, v3, G);
     }
     // ---- b3 basis
     {
@@ -549,12 +582,14 @@ PQ_HD void fsc_end_grad(const Dual<R>& y1, const Dual<R>& y2, const Dual<R>& y3,
         const Dual<R> v3 =
             (R(2) * k2) * Fib + (R(2) * k2 * s) * (y2 * irz * carb) +
             (y2W * s) * irbirz * (R(1) + rc * irz * carb + a * y3b * ir2);
-        fsc_scatter<R, NB>(bS, 2, K, e1x, e1y, v1, v2, v3, G);
+        fsc_emit<R, NB, BASIS>(bS, 2, K, e1x, e1y, v1, v2, v3, G);
     }
 }
 
 // One side of the free-surface correction (AngSetupFSC, DH_greenfunction.py:234-271)
-template <class R, int NB>
+// BASIS: G's three rows take the responses to the unit Burgers vectors along
+// the side-frame axes (bS unused), as in fsc_end_grad.
+template <class R, int NB, bool BASIS = false>
 PQ_HD void fsc_side_grad(const FSCSide& sd, const double (*bS)[3], R X, R Y, R Z, R nu, R (*G)[9]) {
     if (sd.skip) return;
     const R e1x = R(sd.e1x), e1y = R(sd.e1y);
@@ -568,7 +603,7 @@ PQ_HD void fsc_side_grad(const FSCSide& sd, const double (*bS)[3], R X, R Y, R Z
 #pragma unroll 1
     for (int e = 0; e < 2; ++e) {
         const R o0 = e ? R(sd.yAB[0]) : R(0), o1 = e ? R(sd.yAB[1]) : R(0), o2 = e ? R(sd.yAB[2]) : R(0);
-        fsc_end_grad<R, NB>(yA1 - o0, yA2 - o1, yA3 - o2, cs, e ? R(sd.aB) : R(sd.aA), nu, bS, e1x, e1y,
+        fsc_end_grad<R, NB, BASIS>(yA1 - o0, yA2 - o1, yA3 - o2, cs, e ? R(sd.aB) : R(sd.aA), nu, bS, e1x, e1y,
                             e ? R(1) : R(-1), G);
     }
 }
@@ -590,6 +625,105 @@ PQ_HD void td_pair_grad(const SrcGeom<NB>& g, R X, R Y, R Z, R nu, bool halfspac
     if (halfspace) {
 #pragma unroll 1
         for (int s = 0; s < 3; ++s) fsc_side_grad<R, NB>(g.side[s], g.bS[s], X, Y, Z, nu, G);
+    }
+}
+
+// --------------------------------------------------------------------------
+// Sides shared by consecutive sources.  The free-surface correction of a side
+// is linear in the Burgers vector, and that of side BA is minus that of side
+// AB (the angular-dislocation pair traversed the other way); in BA's frame
+// the axes ey1, ey2 are reversed and ey3 kept.  So the responses of side AB
+// to the three unit Burgers vectors of its frame, R[k], serve both sources:
+//   AB's source:  G[b] += sum_k bS_AB[b][k] R[k]
+//   BA's source:  G[b] += bS_BA[b][0] R[0] + bS_BA[b][1] R[1] - bS_BA[b][2] R[2]
+// (a neighbour that runs the side the same way uses its bS unchanged).
+// --------------------------------------------------------------------------
+
+// The side of triangle A (side k runs from vertex k to vertex k + 1 mod 3, as
+// the correction's sides are set up) that triangle B also has, when the two
+// share exactly two vertices (bitwise-equal coordinates); -1 otherwise.
+// rev: B runs that side the other way.
+PQ_HD bool same_point(const double* a, const double* b) { return a[0] == b[0] && a[1] == b[1] && a[2] == b[2]; }
+PQ_HD int shared_side(const double (*A)[3], const double (*B)[3], int* rev) {
+    int match[3] = {-1, -1, -1};  // B's vertex at each of A's vertices
+    int c = 0;
+    for (int a = 0; a < 3; ++a)
+        for (int b = 0; b < 3; ++b)
+            if (same_point(A[a], B[b])) {
+                if (match[a] < 0) match[a] = b;
+                ++c;
+            }
+    *rev = 0;
+    if (c != 2) return -1;
+    for (int k = 0; k < 3; ++k) {
+        const int b0 = match[k], b1 = match[(k + 1) % 3];
+        if (b0 < 0 || b1 < 0 || b0 == b1) continue;
+        *rev = (b1 == (b0 + 1) % 3) ? 0 : 1;
+        return k;
+    }
+    return -1;
+}
+
+// One pair of a run of sources: as td_pair_grad, but the side g.next_side
+// (shared with the next source) is evaluated for the three unit Burgers
+// vectors and handed on through store(k, c, value), and when use_prev is set
+// the side g.prev_side takes the previous source's responses through
+// load(k, c) instead of being evaluated.  The responses are stored before the
+// previous ones are loaded, so a caller can keep two alternating buffers.
+template <class R, class Store, class Load>
+PQ_HD void td_pair_grad_shared(const SrcGeom<2>& g, R X, R Y, R Z, R nu, bool halfspace, bool use_prev,
+                               Store store, Load load, R (*G)[9]) {
+#pragma unroll
+    for (int b = 0; b < 2; ++b)
+#pragma unroll
+        for (int k = 0; k < 9; ++k) G[b][k] = R(0);
+    int ns = -1, ps = -1;
+    if (halfspace) {
+        ns = g.next_side;
+        ps = use_prev ? g.prev_side : -1;
+        if (ns >= 0) {
+            R Rb[3][9];
+#pragma unroll
+            for (int k = 0; k < 3; ++k)
+#pragma unroll
+                for (int c = 0; c < 9; ++c) Rb[k][c] = R(0);
+            fsc_side_grad<R, 3, true>(g.side[ns], nullptr, X, Y, Z, nu, Rb);
+#pragma unroll
+            for (int k = 0; k < 3; ++k)
+#pragma unroll
+                for (int c = 0; c < 9; ++c) store(k, c, Rb[k][c]);
+#pragma unroll
+            for (int b = 0; b < 2; ++b)
+#pragma unroll
+                for (int k = 0; k < 3; ++k) {
+                    const R w = R(g.bS[ns][b][k]);
+#pragma unroll
+                    for (int c = 0; c < 9; ++c) G[b][c] += w * Rb[k][c];
+                }
+        }
+        if (ps >= 0) {
+            const double s2 = g.prev_rev ? -1.0 : 1.0;
+#pragma unroll
+            for (int k = 0; k < 3; ++k)
+#pragma unroll
+                for (int c = 0; c < 9; ++c) {
+                    const R v = load(k, c);
+#pragma unroll
+                    for (int b = 0; b < 2; ++b) {
+                        const R w = R(k == 2 ? s2 * g.bS[ps][b][2] : g.bS[ps][b][k]);
+                        G[b][c] += w * v;
+                    }
+                }
+        }
+    }
+    // main TD, then (half space) its image in z = 0
+#pragma unroll 1
+    for (int t = 0; t < (halfspace ? 2 : 1); ++t)
+        td_fullspace_grad<R, 2, false>(t ? g.image : g.main, X, Y, Z, g.bT, nu, t == 1 && g.flush != 0, G);
+    if (halfspace) {
+#pragma unroll 1
+        for (int sd = 0; sd < 3; ++sd)
+            if (sd != ns && sd != ps) fsc_side_grad<R, 2>(g.side[sd], g.bS[sd], X, Y, Z, nu, G);
     }
 }
 

@@ -8,7 +8,7 @@
 //     with GetTtstress :1400-1417), optionally over row/column index subsets
 //     (the dense leaves and ACA rows/columns of the H-matrix).
 //
-// Work decomposition: one source-receiver pair per lane.  A workgroup owns 256
+// Work decomposition: one source-receiver pair per lane.  A workgroup owns 64
 // receivers (lanes) and walks a chunk of sources; per-source constants come
 // from a precomputed table through wave-uniform (scalar) loads, and the
 // outputs are stored source-major (column-major, element (r, s) at s*ld + r)
@@ -32,6 +32,18 @@ struct Tri {
     double P1[3], P2[3], P3[3];
 };
 
+__device__ __forceinline__ void load_tri(const double* P1, const double* P2, const double* P3, long long s,
+                                         double (*V)[3]) {
+    for (int k = 0; k < 3; ++k) {
+        V[0][k] = P1[3 * s + k];
+        V[1][k] = P2[3 * s + k];
+        V[2][k] = P3[3 * s + k];
+    }
+}
+
+// Table entry j: the source's geometry and Burgers vectors, and the sides it
+// shares with entries j - 1 and j + 1 (the assembly kernel evaluates a shared
+// side's free-surface correction once for both entries).
 template <int NB, class B>
 __global__ void src_setup_kernel(const double* __restrict__ P1, const double* __restrict__ P2,
                                  const double* __restrict__ P3, const long long* __restrict__ cols, long long n,
@@ -41,6 +53,18 @@ __global__ void src_setup_kernel(const double* __restrict__ P1, const double* __
     long long s = cols ? cols[j] : j;
     SrcGeom<NB> g;
     src_setup<NB>(P1 + 3 * s, P2 + 3 * s, P3 + 3 * s, burg.b, g);
+    double V[3][3], W[3][3];
+    load_tri(P1, P2, P3, s, V);
+    if (j > 0) {
+        load_tri(P1, P2, P3, cols ? cols[j - 1] : j - 1, W);
+        g.prev_side = shared_side(V, W, &g.prev_rev);
+    }
+    if (j + 1 < n) {
+        int rev;
+        load_tri(P1, P2, P3, cols ? cols[j + 1] : j + 1, W);
+        g.next_side = shared_side(V, W, &rev);
+    }
+    if (g.prev_side >= 0 && g.prev_side == g.next_side) g.prev_side = -1;  // an edge of three triangles
     out[j] = g;
 }
 
@@ -63,16 +87,26 @@ template <> __device__ __forceinline__ float cvt_out<float>(double x) { return (
 
 // Dense traction Green's functions for receivers rows[0:nr] x sources (setup
 // table entries 0:nc).  Output k in {A1s, A2s, Bs, A1d, A2d, Bd}, element
-// (i, j) at out.m[k][j * ld + i].
+// (i, j) at out.m[k][j * ld + i].  One wave per workgroup (lanes = receivers)
+// walks entries j0 .. j1 - 1 in table order.  A side shared by consecutive
+// entries has its free-surface correction evaluated once: the first entry
+// takes the responses to the three unit Burgers vectors of the side frame
+// (kept per lane in LDS) and weights them by its own Burgers vectors; the
+// next entry weights the same responses by its Burgers vectors, negated for a
+// side it runs the other way (the correction of side BA is minus that of side
+// AB, with the frame axes ey1, ey2 reversed and ey3 kept).  That replaces two
+// of the six angular-dislocation end evaluations of most pairs.
 template <class T>
-__global__ __launch_bounds__(256) void gf_assemble_kernel(const SrcGeom<2>* __restrict__ geo,
-                                                          const double* __restrict__ xg,
-                                                          const double* __restrict__ ev,
-                                                          const long long* __restrict__ rows, long long nr,
-                                                          long long nc, int src_per_block, double mu,
-                                                          double lam, double nu, int halfspace, Mat6<T> out,
-                                                          long long ld) {
-    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+__global__ __launch_bounds__(64) void gf_assemble_kernel(const SrcGeom<2>* __restrict__ geo,
+                                                         const double* __restrict__ xg,
+                                                         const double* __restrict__ ev,
+                                                         const long long* __restrict__ rows, long long nr,
+                                                         long long nc, int src_per_block, double mu,
+                                                         double lam, double nu, int halfspace, Mat6<T> out,
+                                                         long long ld) {
+    __shared__ double carry[2][27][64];  // [entry parity][basis k * 9 + gradient component][lane]
+    const int lane = threadIdx.x;
+    const long long i = (long long)blockIdx.x * 64 + lane;
     const bool active = i < nr;
     const long long r = active ? (rows ? rows[i] : i) : 0;
     const double X = xg[3 * r + 0], Y = xg[3 * r + 1], Z = xg[3 * r + 2];
@@ -82,8 +116,12 @@ __global__ __launch_bounds__(256) void gf_assemble_kernel(const SrcGeom<2>* __re
     const long long j0 = (long long)blockIdx.y * src_per_block;
     const long long j1 = j0 + src_per_block < nc ? j0 + src_per_block : nc;
     for (long long j = j0; j < j1; ++j) {
+        const int cur = (int)((j - j0) & 1);
         double G[2][9];
-        td_pair_grad<double, 2, false>(geo[j], X, Y, Z, nu, halfspace != 0, G);
+        td_pair_grad_shared<double>(
+            geo[j], X, Y, Z, nu, halfspace != 0, j > j0,
+            [&](int k, int c, double v) { carry[cur][9 * k + c][lane] = v; },
+            [&](int k, int c) { return carry[cur ^ 1][9 * k + c][lane]; }, G);
 #pragma unroll
         for (int b = 0; b < 2; ++b) {
             double eps[6], sig[6], t[3];
@@ -214,7 +252,18 @@ static int gf_assemble_check(GfAssembleArgs& a) {
     for (long long j = 0; a.cols && j < a.nc; ++j)
         PQ_CHECK(a.cols[j] >= 0 && a.cols[j] < a.nsrc, "pq3d_gf_assemble: column index %lld out of range",
                  a.cols[j]);
-    if (a.src_per_block <= 0) a.src_per_block = 16;
+    if (a.src_per_block <= 0) {
+        // the longest source runs (a run's first source shares no side with a
+        // predecessor) that still give the grid 8192 workgroups: eight
+        // one-wave workgroups per SIMD of the 256-CU part
+        const long long gx = (a.nr + 63) / 64;
+        a.src_per_block = 4;
+        for (int spb = 64; spb > 4; spb /= 2)
+            if (gx * ((a.nc + spb - 1) / spb) >= 8192) {
+                a.src_per_block = spb;
+                break;
+            }
+    }
     return 0;
 }
 
@@ -248,17 +297,17 @@ static int gf_assemble_staged(const GfAssembleArgs& a, char* base) {
         hipLaunchKernelGGL((src_setup_kernel<2, Burgers2>), dim3((unsigned)((nc + 255) / 256)), dim3(256), 0, 0,
                            d_P, d_P + 3 * nsrc, d_P + 6 * nsrc, a.cols ? d_cols : nullptr, nc, b, d_geo);
         const int spb = a.src_per_block;
-        dim3 grid((unsigned)((nr + 255) / 256), (unsigned)((nc + spb - 1) / spb));
+        dim3 grid((unsigned)((nr + 63) / 64), (unsigned)((nc + spb - 1) / spb));
         const double nu = poisson(a.mu, a.lam);
         if (a.out_type == MAT_F64) {
             Mat6<double> o;
             for (int k = 0; k < 6; ++k) o.m[k] = (double*)a.d_out[k];
-            hipLaunchKernelGGL(gf_assemble_kernel<double>, grid, dim3(256), 0, 0, d_geo, d_xg, d_ev,
+            hipLaunchKernelGGL(gf_assemble_kernel<double>, grid, dim3(64), 0, 0, d_geo, d_xg, d_ev,
                                a.rows ? d_rows : nullptr, nr, nc, spb, a.mu, a.lam, nu, a.halfspace, o, a.ld);
         } else {
             Mat6<float> o;
             for (int k = 0; k < 6; ++k) o.m[k] = (float*)a.d_out[k];
-            hipLaunchKernelGGL(gf_assemble_kernel<float>, grid, dim3(256), 0, 0, d_geo, d_xg, d_ev,
+            hipLaunchKernelGGL(gf_assemble_kernel<float>, grid, dim3(64), 0, 0, d_geo, d_xg, d_ev,
                                a.rows ? d_rows : nullptr, nr, nc, spb, a.mu, a.lam, nu, a.halfspace, o, a.ld);
         }
         e = hipGetLastError();

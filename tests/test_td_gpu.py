@@ -120,7 +120,38 @@ def test_gf_assembly_fp32_and_subblocks(golden_td):
     rows = rng.permutation(len(xg))[:17]
     cols = rng.permutation(len(P1))[:23]
     sub = np.stack(gf.traction_gf(P1, P2, P3, xg, ev, MU, LAM, True, rows=rows, cols=cols))
-    assert np.array_equal(sub, full[:, rows][:, :, cols])
+    # the full assembly evaluates a side shared by consecutive columns once for
+    # both, a subset of scattered columns evaluates every side: rounding apart
+    assert (np.abs(sub - full[:, rows][:, :, cols]) / col[cols]).max() < 1e-12
+    # the same columns in their mesh order, as one run, share their sides again
+    run = np.arange(40, 63)
+    sub = np.stack(gf.traction_gf(P1, P2, P3, xg, ev, MU, LAM, True, rows=rows, cols=run))
+    assert (np.abs(sub - full[:, rows][:, :, run]) / col[run]).max() < 1e-12
+
+
+def test_gf_assembly_shared_sides_both_orientations():
+    """A non-planar mesh where consecutive triangles share sides run both ways
+    (every third triangle's vertex order reversed, so some neighbours run a
+    shared side the same way): every pair against the long-double oracle,
+    with the side sharing on (mesh order) and off (columns in reverse order
+    share sides the same way as forward; shuffled columns share none)."""
+    from pyquake3d_amd import readmsh, synth
+    gf = _gf()
+    nodes_km, tri = synth.rough_fault_mesh(12, 6, length_km=12.0, depth_km=6.0, amp_km=0.5,
+                                           wavelength_km=6.0, seed=4)
+    tri = np.array(tri)
+    tri[::3] = tri[::3, ::-1]
+    nodes = nodes_km * 1e3
+    ev, xg = readmsh.get_eleVec(nodes, tri, False)
+    P1, P2, P3 = gf.source_vertices(nodes, tri)
+    n = len(tri)
+    orc = np.stack(T.traction_gf(nodes, tri, MU, LAM, True))
+    col = np.abs(orc).max(axis=(0, 1))
+    got = np.stack(gf.traction_gf(P1, P2, P3, xg, ev, MU, LAM, True))
+    assert (np.abs(got - orc) / col).max() < 1e-12
+    for cols in (np.arange(n)[::-1], np.random.default_rng(5).permutation(n)):
+        sub = np.stack(gf.traction_gf(P1, P2, P3, xg, ev, MU, LAM, True, cols=cols))
+        assert (np.abs(sub - orc[:, :, cols]) / col[cols]).max() < 1e-12
 
 
 def test_gf_assembly_full_space_and_empty(golden_td):

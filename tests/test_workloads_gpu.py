@@ -387,15 +387,21 @@ def test_20k_rupturing_members_follow_fp64_dense_events(mesh20k):
     (nucleation size near 7 km, so they rupture; the metric's members, with
     a - b in [-0.006, -0.002], creep), rake 0.5, on the 20,000-element mesh,
     FP32 matrices and the BF16-split GEMM, are stepped until four members
-    have completed their first event; each of those four is then run alone
-    through the plain FP64 dense path on the FP64-assembled matrices for as
-    many accepted steps.  Bars: the same first event, onset within 1e-3
-    relative and peak slip rate within 1 %.  Rake 0.5 because at rake 0 the
-    dip traction is rounding noise and the relative error estimate of it
-    steers the step sizes: measured there, scaling the FP64 run's initial
-    state by 1 + 1e-12 moved its first event by 28 %, while at rake 0.5 the
-    same perturbation moves it by 1e-5 to 1e-4.  Measured: onsets 8e-6 to
-    7e-5 from the FP64 runs, peaks within 0.6 %."""
+    have completed a first event that became a dynamic rupture (peak slip
+    rate above 0.1 m/s); each of those four is then run alone through the
+    plain FP64 dense path on the FP64-assembled matrices for as many accepted
+    steps.  Bars: the same first event, onset within 1e-3 relative and peak
+    slip rate within 1 %.  Rake 0.5 because at rake 0 the dip traction is
+    rounding noise and the relative error estimate of it steers the step
+    sizes: measured there, scaling the FP64 run's initial state by 1 + 1e-12
+    moved its first event by 28 %, while at rake 0.5 the same perturbation
+    moves it by 1e-5 to 1e-4.  A first event that stalls near 1e-2 m/s is
+    left out: it is not a rupture, and its peak, exponential in the traction
+    there (V ~ exp(tau / (a sigma))), moves by percents with the FP32 storage
+    rounding (one measured at 0.016 m/s had its onset within 3.6e-5 of the
+    FP64 run's and its peak 2.9 % off).
+    Measured on ruptures: onsets 8e-6 to 7e-5 from the FP64 runs, peaks
+    within 0.6 %."""
     from pyquake3d_amd import _lib, synth
     from pyquake3d_amd.QDsim_gpu import DenseSim
     from pyquake3d_amd.ensemble import Ensemble, init_members, steady_state_fields
@@ -408,10 +414,10 @@ def test_20k_rupturing_members_follow_fp64_dense_events(mesh20k):
     init_members(e, cols, V0, VPL, MU, CS)
     recs = [[] for _ in range(B)]
     last = np.zeros(B, dtype=np.int64)
-    done = []
+    done, stalled = [], []
     g = 0
     while len(done) < 4:
-        assert g < 12000, f"only {len(done)} members completed an event"
+        assert g < 12000, f"only {len(done)} members completed a rupture ({len(stalled)} stalled events)"
         e.run(1, graph_steps=1)
         g += 1
         mem = e.members()
@@ -422,10 +428,14 @@ def test_20k_rupturing_members_follow_fp64_dense_events(mesh20k):
         last = mem["steps"].copy()
         if g % 25 == 0:
             for m in range(B):
-                if m not in done and recs[m] and _first_event_complete(np.array(recs[m])[:, 1]):
-                    done.append(m)
+                if m in done or m in stalled or not recs[m]:
+                    continue
+                r = np.array(recs[m])
+                if _first_event_complete(r[:, 1]):
+                    (stalled if _events(r[:, 0], r[:, 1])[0][1] < 0.1 else done).append(m)
     e.close()
-    print(f"{g} global steps; members with a complete first event: {done}", flush=True)
+    print(f"{g} global steps; members with a complete first rupture: {done}; first event stalled: {stalled}",
+          flush=True)
     for m in done[:4]:
         r = np.array(recs[m])
         s64 = DenseSim(n, _lib.MAT_F64)
