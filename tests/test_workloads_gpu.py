@@ -400,8 +400,8 @@ def test_20k_rupturing_members_follow_fp64_dense_events(mesh20k):
     there (V ~ exp(tau / (a sigma))), moves by percents with the FP32 storage
     rounding (one measured at 0.016 m/s had its onset within 3.6e-5 of the
     FP64 run's and its peak 2.9 % off).
-    Measured on ruptures: onsets 8e-6 to 7e-5 from the FP64 runs, peaks
-    within 0.6 %."""
+    Measured on ruptures (peaks 0.11-0.30 m/s): onsets 2e-5 to 1.2e-4 from
+    the FP64 runs, peaks within 0.1 %."""
     from pyquake3d_amd import _lib, synth
     from pyquake3d_amd.QDsim_gpu import DenseSim
     from pyquake3d_amd.ensemble import Ensemble, init_members, steady_state_fields
