@@ -80,7 +80,6 @@ struct HMat {
     int ftype = MAT_F32;
     int vec = 4;
     int* perm = nullptr;      // cluster position -> element id
-    int* row_seg = nullptr;   // cluster position -> segment
     AItem* a_items = nullptr;
     long long na = 0;
     BItem* b_items = nullptr;
@@ -409,42 +408,50 @@ __global__ __launch_bounds__(256) void hmat_lowrank_kernel(const BWave* __restri
     b_store<VR>(acc, VR, qp, q, g, ms, partial + wv.slot);
 }
 
-// reduce: per cluster-order row, eight lanes each sum every eighth item of
-// the row's segment, then the eight partial sums are combined in a fixed
-// butterfly (deterministic); FP64 sums; scattered to element order:
-// out[q * n + perm[p]]
+// reduce: one workgroup of four waves per segment; lane i of wave v sums, in
+// FP64, the partial rows of the segment's phase-B waves v, v + 4, ... for row
+// i (each load reads one wave's partial row block contiguously), and the four
+// wave sums are added in wave order (deterministic: results do not depend on
+// scheduling); scattered to element order: out[q * n + perm[p]]
 __global__ __launch_bounds__(256) void hmat_reduce_kernel(const Seg* __restrict__ segs,
-                                                          const int* __restrict__ row_seg,
                                                           const int* __restrict__ perm, long long n,
                                                           const float* __restrict__ partial,
                                                           double* __restrict__ out) {
-    const long long tid = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    const long long p = tid >> 3;
-    const int sub = (int)(tid & 7);
-    const bool ok = p < n;
+    __shared__ double part[4][3][64];
+    const Seg s = segs[blockIdx.x];
+    const int i = threadIdx.x & 63, v = threadIdx.x >> 6;
+    const bool ok = i < s.ms;
     double y0 = 0, y1 = 0, y2 = 0;
     if (ok) {
-        const Seg s = segs[row_seg[p]];
-        const int i = (int)(p - s.s0);
         const long long stride = 3LL * s.ms;
-        const float* P = partial + s.slot0 + i + stride * sub;
-        for (int it = sub; it < s.nw; it += 8, P += 8 * stride) {
+        const float* P = partial + s.slot0 + i + stride * v;
+        int w = v;
+        // two waves' rows in flight per step
+        for (; w + 4 < s.nw; w += 8, P += 8 * stride) {
+            const float a0 = P[0], a1 = P[s.ms], a2 = P[2 * s.ms];
+            const float b0 = P[4 * stride], b1 = P[4 * stride + s.ms], b2 = P[4 * stride + 2 * s.ms];
+            y0 += (double)a0;
+            y1 += (double)a1;
+            y2 += (double)a2;
+            y0 += (double)b0;
+            y1 += (double)b1;
+            y2 += (double)b2;
+        }
+        if (w < s.nw) {
             y0 += (double)P[0];
             y1 += (double)P[s.ms];
             y2 += (double)P[2 * s.ms];
         }
     }
+    part[v][0][i] = y0;
+    part[v][1][i] = y1;
+    part[v][2][i] = y2;
+    __syncthreads();
+    if (v == 0 && ok) {
+        const int e = perm[s.s0 + i];
 #pragma unroll
-    for (int o = 1; o < 8; o <<= 1) {
-        y0 += __shfl_xor(y0, o, 64);
-        y1 += __shfl_xor(y1, o, 64);
-        y2 += __shfl_xor(y2, o, 64);
-    }
-    if (ok && sub == 0) {
-        const int e = perm[p];
-        out[e] = y0;
-        out[n + e] = y1;
-        out[2 * n + e] = y2;
+        for (int c = 0; c < 3; ++c)
+            out[c * n + e] = ((part[0][c][i] + part[1][c][i]) + part[2][c][i]) + part[3][c][i];
     }
 }
 
@@ -471,8 +478,7 @@ static void launch_mvm_t(HMat* h, const double* V1, const double* V2, const doub
     if (h->nbw > h->nbd)
         L(hmat_lowrank_kernel<T>, dim3((unsigned)((h->nbw - h->nbd + 3) / 4)), dim3(256), h->b_waves + h->nbd,
           h->nbw - h->nbd, h->b_items, (const T*)h->udata, h->t, h->partial, fix_tn);
-    L(hmat_reduce_kernel, dim3((unsigned)((8 * n + 255) / 256)), dim3(256), h->segs, h->row_seg, h->perm, n,
-      h->partial, out);
+    if (h->nseg) L(hmat_reduce_kernel, dim3((unsigned)h->nseg), dim3(256), h->segs, h->perm, n, h->partial, out);
 }
 
 // Traction rates of one step into out (3 x n, element order): x1 = V1 - Vpl
@@ -761,9 +767,6 @@ static int pack(HMat* h, long long n, const int* perm, long long nb, const long 
     std::vector<BWave> bwaves(dwaves);
     bwaves.insert(bwaves.end(), lwaves.begin(), lwaves.end());
     PQ_CHECK(slot < INT32_MAX && tcount < INT32_MAX, "pq3d_hmat_create: H-matrix too large for 32-bit offsets");
-    std::vector<int> row_seg(n);
-    for (size_t s = 0; s < segs.size(); ++s)
-        for (int p = segs[s].s0; p < segs[s].s0 + segs[s].ms; ++p) row_seg[p] = (int)s;
     // upload
     h->n = n;
     h->na = (long long)aitems.size();
@@ -779,7 +782,6 @@ static int pack(HMat* h, long long n, const int* perm, long long nb, const long 
         return 0;
     };
     if (up((void**)&h->perm, perm, n * sizeof(int))) return -1;
-    if (up((void**)&h->row_seg, row_seg.data(), n * sizeof(int))) return -1;
     if (up((void**)&h->a_items, aitems.data(), aitems.size() * sizeof(AItem))) return -1;
     if (up((void**)&h->b_items, bitems.data(), bitems.size() * sizeof(BItem))) return -1;
     if (up((void**)&h->b_waves, bwaves.data(), bwaves.size() * sizeof(BWave))) return -1;
@@ -798,7 +800,7 @@ static int pack(HMat* h, long long n, const int* perm, long long nb, const long 
 }
 
 static void hmat_free(HMat* h) {
-    void* ptrs[] = {h->perm, h->row_seg, h->a_items, h->b_items, h->b_waves, h->segs, h->ddata, h->udata,
+    void* ptrs[] = {h->perm, h->a_items, h->b_items, h->b_waves, h->segs, h->ddata, h->udata,
                     h->vdata, h->xp, h->t, h->partial, h->xin, h->yout};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
