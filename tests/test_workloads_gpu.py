@@ -381,31 +381,17 @@ def _first_event_complete(v, on=1e-3, off=1e-4):
     return len(above) > 0 and bool(np.any(v[above[0]:] < off))
 
 
-def test_20k_rupturing_members_follow_fp64_dense_events(mesh20k):
-    """Seismic events of batched members at the ensemble's own size, against
-    the plain FP64 path.  64 members with a - b uniform in [-0.026, -0.022]
-    (nucleation size near 7 km, so they rupture; the metric's members, with
-    a - b in [-0.006, -0.002], creep), rake 0.5, on the 20,000-element mesh,
-    FP32 matrices and the BF16-split GEMM, are stepped until four members
-    have completed a first event that became a dynamic rupture (peak slip
-    rate above 0.1 m/s); each of those four is then run alone through the
-    plain FP64 dense path on the FP64-assembled matrices for as many accepted
-    steps.  Bars: the same first event, onset within 1e-3 relative and peak
-    slip rate within 1 %.  Rake 0.5 because at rake 0 the dip traction is
-    rounding noise and the relative error estimate of it steers the step
-    sizes: measured there, scaling the FP64 run's initial state by 1 + 1e-12
-    moved its first event by 28 %, while at rake 0.5 the same perturbation
-    moves it by 1e-5 to 1e-4.  A first event that stalls near 1e-2 m/s is
-    left out: it is not a rupture, and its peak, exponential in the traction
-    there (V ~ exp(tau / (a sigma))), moves by percents with the FP32 storage
-    rounding (one measured at 0.016 m/s had its onset within 3.6e-5 of the
-    FP64 run's and its peak 2.9 % off).
-    Measured on ruptures (peaks 0.11-0.30 m/s): onsets 2e-5 to 1.2e-4 from
-    the FP64 runs, peaks within 0.1 %."""
+def _ruptures_follow_fp64(s32, P, xg, ev, max_gsteps):
+    """Step 64 rupturing members (a - b uniform in [-0.026, -0.022], rake
+    0.5) on the FP32 matrices of s32 with the default (BF16-split) GEMM until
+    four have completed a first event that became a rupture (peak above
+    0.1 m/s; a first event that stalls below is set aside), then run each of
+    those four alone through the plain FP64 dense path on FP64-assembled
+    matrices for as many accepted steps, and compare the first events:
+    onset within 1e-3 relative, peak slip rate within 1 %."""
     from pyquake3d_amd import _lib, synth
     from pyquake3d_amd.QDsim_gpu import DenseSim
     from pyquake3d_amd.ensemble import Ensemble, init_members, steady_state_fields
-    s32, P, xg, ev, _ = mesh20k
     n = s32.n
     cols = np.stack([synth.member_params(xg, m, base_seed=5000, amb_range=(-0.026, -0.022), rake=0.5)
                      for m in range(B)])
@@ -417,7 +403,7 @@ def test_20k_rupturing_members_follow_fp64_dense_events(mesh20k):
     done, stalled = [], []
     g = 0
     while len(done) < 4:
-        assert g < 12000, f"only {len(done)} members completed a rupture ({len(stalled)} stalled events)"
+        assert g < max_gsteps, f"only {len(done)} members completed a rupture ({len(stalled)} stalled events)"
         e.run(1, graph_steps=1)
         g += 1
         mem = e.members()
@@ -456,3 +442,51 @@ def test_20k_rupturing_members_follow_fp64_dense_events(mesh20k):
               f"peak V {ve:.4f} / {vd:.4f} m/s", flush=True)
         assert abs(te - td) <= 1e-3 * td
         assert abs(ve - vd) <= 1e-2 * vd
+
+
+def test_20k_rupturing_members_follow_fp64_dense_events(mesh20k):
+    """Seismic events of batched members at the ensemble's own size, against
+    the plain FP64 path.  64 members with a - b uniform in [-0.026, -0.022]
+    (nucleation size near 7 km, so they rupture; the metric's members, with
+    a - b in [-0.006, -0.002], creep), rake 0.5, on the 20,000-element mesh,
+    FP32 matrices and the BF16-split GEMM, are stepped until four members
+    have completed a first event that became a dynamic rupture (peak slip
+    rate above 0.1 m/s); each of those four is then run alone through the
+    plain FP64 dense path on the FP64-assembled matrices for as many accepted
+    steps.  Bars: the same first event, onset within 1e-3 relative and peak
+    slip rate within 1 %.  Rake 0.5 because at rake 0 the dip traction is
+    rounding noise and the relative error estimate of it steers the step
+    sizes: measured there, scaling the FP64 run's initial state by 1 + 1e-12
+    moved its first event by 28 %, while at rake 0.5 the same perturbation
+    moves it by 1e-5 to 1e-4.  A first event that stalls near 1e-2 m/s is
+    left out: it is not a rupture, and its peak, exponential in the traction
+    there (V ~ exp(tau / (a sigma))), moves by percents with the FP32 storage
+    rounding (one measured at 0.016 m/s had its onset within 3.6e-5 of the
+    FP64 run's and its peak 2.9 % off).
+    Measured on ruptures (peaks 0.11-0.30 m/s): onsets 2e-5 to 1.2e-4 from
+    the FP64 runs, peaks within 0.1 %."""
+    s32, P, xg, ev, _ = mesh20k
+    _ruptures_follow_fp64(s32, P, xg, ev, max_gsteps=12000)
+
+
+def test_rough_fault_rupturing_members_follow_fp64_dense_events():
+    """The same comparison on a non-planar fault, where the normal traction
+    moves with slip and the split GEMM runs its normal pass (Bs, Bd nonzero):
+    3,200 elements of about 1 km (40 x 40 km, the velocity-weakening patch
+    inside it), fault-normal offset 1 km sin(2 pi x / 20 km) plus
+    self-affine roughness.  Same members and bars as the 20k test.
+    Measured (peaks 0.14-0.29 m/s, about 1,800 global steps): onsets 2.7e-5
+    to 8.6e-5 from the FP64 runs, peaks within 0.07 %."""
+    from pyquake3d_amd import _lib, greenfunc, readmsh, synth
+    from pyquake3d_amd.QDsim_gpu import DenseSim
+    nodes_km, tri = synth.rough_fault_mesh(40, 40, length_km=40.0, depth_km=40.0, amp_km=1.0,
+                                           wavelength_km=20.0, seed=5)
+    nodes = nodes_km * 1e3
+    ev, xg = readmsh.get_eleVec(nodes, tri, False)
+    P = greenfunc.source_vertices(nodes, tri)
+    s32 = DenseSim(len(tri), _lib.MAT_F32)
+    s32.assemble(*P, xg, ev, MU, LAM, True)
+    mats = [s32.download_matrix(k) for k in (2, 5)]
+    assert np.abs(mats[0]).max() > 0 and np.abs(mats[1]).max() > 0   # Bs, Bd nonzero
+    _ruptures_follow_fp64(s32, P, xg, ev, max_gsteps=20000)
+    s32.close()
