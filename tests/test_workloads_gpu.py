@@ -14,7 +14,9 @@
     steps, step sizes within 1e-6;
   - seismic events of batched members (64 rupturing members, BF16-split
     GEMM) against the plain FP64 dense path run alone for four of them:
-    first-event onset within 1e-3 relative, peak slip rate within 1 %.
+    first-rupture onset within 1e-3 relative, peak slip rate within 1 %;
+    the same on a 3,200-element rough fault (normal traction moving, the
+    split GEMM's normal pass).
 * 60,000-element rough fault, H-matrix with leaf 64, eta 2, ACA 1e-4:
   - per block class (dense leaves; low-rank blocks by row count <= 256,
     257-2048, > 2048), 40 sampled blocks checked on random <= 128 x 128
