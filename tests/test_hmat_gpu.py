@@ -18,7 +18,9 @@ element size, leaf 32, eta 2, ACA tolerance 1e-4.  Bars:
   zero;
 * stepping: a simulation whose traction rates come from the H-matrix follows
   the NumPy oracle run on the H-matrix's own dense expansion (FP32-rounded
-  factors) to 1e-6 over 30 steps.
+  factors) to 1e-6 over 30 steps; and through a seismic event (3,200-element
+  rough fault, leaf 64) its first rupture follows the exact FP64 dense
+  operator's: onset within 1e-3, peak within 1 %.
 """
 import numpy as np
 import pytest
