@@ -16,7 +16,8 @@
     GEMM) against the plain FP64 dense path run alone for four of them:
     first-rupture onset within 1e-3 relative, peak slip rate within 1 %;
     the same on a 3,200-element rough fault (normal traction moving, the
-    split GEMM's normal pass).
+    split GEMM's normal pass); and there the plain single simulation with
+    FP32 Green's functions against FP64 storage through one rupture.
 * 60,000-element rough fault, H-matrix with leaf 64, eta 2, ACA 1e-4:
   - per block class (dense leaves; low-rank blocks by row count <= 256,
     257-2048, > 2048), 40 sampled blocks checked on random <= 128 x 128
@@ -492,3 +493,40 @@ def test_rough_fault_rupturing_members_follow_fp64_dense_events():
     assert np.abs(mats[0]).max() > 0 and np.abs(mats[1]).max() > 0   # Bs, Bd nonzero
     _ruptures_follow_fp64(s32, P, xg, ev, max_gsteps=20000)
     s32.close()
+
+
+def test_rough_fault_fp32_dense_follows_fp64_events():
+    """The plain single-simulation path with FP32 Green's functions through
+    a seismic event, against FP64 storage: one rupturing member (a - b near
+    -0.024, rake 0.5) on the 3,200-element rough fault, 2,500 steps each;
+    the first rupture's onset within 1e-3 relative and peak slip rate within
+    1 % (measured: onset 6.6e-5, peak within 1e-4)."""
+    from pyquake3d_amd import _lib, greenfunc, readmsh, synth
+    from pyquake3d_amd.QDsim_gpu import DenseSim
+    from pyquake3d_amd.ensemble import steady_state_fields
+    nodes_km, tri = synth.rough_fault_mesh(40, 40, length_km=40.0, depth_km=40.0, amp_km=1.0,
+                                           wavelength_km=20.0, seed=5)
+    nodes = nodes_km * 1e3
+    ev, xg = readmsh.get_eleVec(nodes, tri, False)
+    P = greenfunc.source_vertices(nodes, tri)
+    fields = steady_state_fields(synth.member_params(xg, 24, base_seed=5000, amb_range=(-0.026, -0.022), rake=0.5),
+                                 V0, VPL)
+    runs = {}
+    for mt in (_lib.MAT_F32, _lib.MAT_F64):
+        s = DenseSim(len(tri), mt)
+        s.assemble(*P, xg, ev, MU, LAM, True)
+        for k, v in fields.items():
+            s.set(k, v)
+        s.set_params(V0, VPL, MU / (2 * CS), False, True)
+        h, _, _ = s.step(1e-3)
+        st, rec = s.run(2499, 8, records=True)
+        s.close()
+        assert st == 0
+        runs[mt] = _events(np.concatenate(([h], rec[:, 2])), np.concatenate(([0.0], rec[:, 3])))
+    e32, e64 = runs[_lib.MAT_F32], runs[_lib.MAT_F64]
+    assert len(e64) >= 1 and e64[0][1] > 0.1 and len(e32) >= 1
+    (t32, v32), (t64, v64) = e32[0], e64[0]
+    print(f"FP32 storage: onset {t32:.6e} / {t64:.6e} s (rel {abs(t32 - t64) / t64:.2e}), "
+          f"peak {v32:.4f} / {v64:.4f} m/s", flush=True)
+    assert abs(t32 - t64) <= 1e-3 * t64
+    assert abs(v32 - v64) <= 1e-2 * v64
