@@ -264,6 +264,8 @@ static int gf_assemble_check(GfAssembleArgs& a) {
                 break;
             }
     }
+    // the grid's second dimension holds at most 65,535 runs
+    if ((a.nc + a.src_per_block - 1) / a.src_per_block > 65535) a.src_per_block = (int)((a.nc + 65534) / 65535);
     return 0;
 }
 
