@@ -491,6 +491,7 @@ def hmatrix_leg(steps=200, warmup=10, graph_steps=8, cpu_budget_s=20.0):
         "stepping_dense_fp64_steps_per_s": d_rate,
         "dense_fp64_matvec_ms": mv64_ms,
         "dense_fp64_assembly_ms": asm64_ms,
+        "dense_fp64_assembly_pairs_per_s": float(N) * N / (asm64_ms * 1e-3),
         "product_speedup_vs_dense_fp64": mv64_ms / bf["ms"],
         "cpu_baseline": None if cpu_s is None else {
             "value": cpu_s, "unit": "s per step (6 block products)", "cores": 1, "kind": "port",
