@@ -113,17 +113,17 @@ __global__ __launch_bounds__(256) void matvec_kernel(MatPtrs<T> M, long long ld,
     const bool active = r0 < n;
     if (active) {
         long long j = j0;
-        for (; j + 1 < j1; j += 2) {
-            VT a[2][6];
+        for (; j + 3 < j1; j += 4) {  // four columns' loads in flight; the sums keep column order
+            VT a[4][6];
 #pragma unroll
-            for (int t = 0; t < 2; ++t)
+            for (int t = 0; t < 4; ++t)
 #pragma unroll
                 for (int k = 0; k < 6; ++k) {
                     if (FIXTN && (k == 2 || k == 5)) continue;
                     a[t][k] = *reinterpret_cast<const VT*>(M.m[k] + (size_t)(j + t) * ld + r0);
                 }
 #pragma unroll
-            for (int t = 0; t < 2; ++t) {
+            for (int t = 0; t < 4; ++t) {
                 const double u1 = __dsub_rn(V1[j + t], __dmul_rn(Vpl, c0[j + t]));
                 const double u2 = __dsub_rn(V2[j + t], __dmul_rn(Vpl, s0[j + t]));
                 double x[6][VN];
